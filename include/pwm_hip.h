@@ -1,0 +1,195 @@
+/* include/pwm_hip.h -- C-ABI of the MI355X (gfx950) wavelet matrix / wavelet
+ * tree construction engine.
+ *
+ * Plain pointers and sizes only.  Every call returns an int status (0 = OK,
+ * negative = error, see pwmhip_status_string), never aborts and never throws
+ * across the ABI.  Each context owns one HIP stream, a device scratch arena
+ * and an autotune table; calls on distinct contexts are thread-safe, calls on
+ * one context must be serialised by the caller.
+ *
+ * Output layout (identical to the reference's bit_vectors contract):
+ *   level l holds level_bits[l] bits in ceil(level_bits[l]/64) uint64 words;
+ *   bit j lives in word j>>6 at mask 1<<(63-(j&63)) (MSB first); bits past
+ *   level_bits[l] in the last word are 0.
+ *   reference: include/util/common.hpp:60-68,
+ *              include/construction/building_blocks.hpp:18-50,
+ *              include/arrays/flat_two_dim_array.hpp:110-213.
+ */
+#ifndef PWM_HIP_H
+#define PWM_HIP_H
+
+#include <stdint.h>
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+#define PWMHIP_MAX_LEVELS 64
+
+/* status codes */
+#define PWMHIP_OK 0
+#define PWMHIP_ERR_INVALID -1      /* bad argument (levels == 0, width, ...) */
+#define PWMHIP_ERR_NO_DEVICE -2    /* no HIP device / device index invalid  */
+#define PWMHIP_ERR_OOM -3          /* hipMalloc failed                      */
+#define PWMHIP_ERR_HIP -4          /* other HIP runtime error               */
+#define PWMHIP_ERR_SYMBOL_RANGE -5 /* a symbol >= 2^levels                  */
+#define PWMHIP_ERR_UNSUPPORTED -6  /* input the reference leaves undefined  */
+#define PWMHIP_ERR_BUDGET -7       /* request exceeds the device budget     */
+
+/* Construction algorithms.  They produce identical level bits; they differ
+ * only where the reference's outputs differ:
+ *   - zeros[levels-1] is 0 for PPC, PPC_SS, PC_SS, DD_PC_SS (those never
+ *     compute it: construction/ppc.hpp:19-90, pc_ss.hpp:13-33,
+ *     ppc_ss.hpp:19-70) and the true count for the others;
+ *   - the HUFF_* algorithms build Huffman-shaped structures and ignore
+ *     `levels` (wx_huff_pc.hpp:45-80).  HUFF_PC / HUFF_PC_SS / HUFF_DD_PC
+ *     reproduce huff_pc's border bookkeeping (zeros included); HUFF_NAIVE
+ *     gives huff_naive's true zero counts. */
+enum pwmhip_algo {
+  PWMHIP_ALGO_PC = 0,
+  PWMHIP_ALGO_PS = 1,
+  PWMHIP_ALGO_PPC = 2,
+  PWMHIP_ALGO_PPS = 3,
+  PWMHIP_ALGO_PC_SS = 4,
+  PWMHIP_ALGO_PPC_SS = 5,
+  PWMHIP_ALGO_DD_PC = 6,
+  PWMHIP_ALGO_DD_PS = 7,
+  PWMHIP_ALGO_DD_PC_SS = 8,
+  PWMHIP_ALGO_NAIVE = 9,
+  PWMHIP_ALGO_HUFF_PC = 10,
+  PWMHIP_ALGO_HUFF_PS = 11,
+  PWMHIP_ALGO_HUFF_PC_SS = 12,
+  PWMHIP_ALGO_HUFF_NAIVE = 13,
+  PWMHIP_ALGO_HUFF_DD_PC = 14
+};
+
+/* build flags */
+#define PWMHIP_FLAG_TREE 1u            /* wavelet tree instead of matrix       */
+#define PWMHIP_FLAG_TEXT_ON_DEVICE 2u  /* `text` is a device pointer           */
+#define PWMHIP_FLAG_CHECK_RANGE 4u     /* verify all symbols < 2^levels        */
+#define PWMHIP_FLAG_REUSE_OUTPUT 8u    /* `out` holds an earlier result whose   */
+                                       /* device memory is reused if it fits   */
+
+typedef struct pwmhip_ctx pwmhip_ctx;
+
+/* Result of a construction: device-resident levels plus host metadata. */
+typedef struct pwmhip_ws {
+  uint64_t levels;
+  uint64_t text_size;
+  int32_t is_tree;
+  int32_t is_huffman;
+  uint64_t* d_words;                          /* device allocation         */
+  uint64_t level_offset[PWMHIP_MAX_LEVELS];   /* word offset of level l    */
+  uint64_t level_bits[PWMHIP_MAX_LEVELS];     /* bits in level l           */
+  uint64_t n_zeros;                           /* = levels for matrices     */
+  uint64_t zeros[PWMHIP_MAX_LEVELS];
+  uint64_t n_codes;                           /* Huffman: max_symbol + 1   */
+  uint64_t* code_len;                         /* host arrays (n_codes)     */
+  uint64_t* code_word;
+  uint64_t alloc_bytes;
+} pwmhip_ws;
+
+/* ---- context ------------------------------------------------------------
+ * Replaces nothing in the reference (it has no device state); every
+ * wx_*::compute call of the reference (include/wx_*.hpp) becomes
+ * pwmhip_build on a context. */
+int pwmhip_create(pwmhip_ctx** ctx, int device);
+int pwmhip_destroy(pwmhip_ctx* ctx);
+const char* pwmhip_status_string(int status);
+int pwmhip_device_count(int* count);
+
+/* ---- construction --------------------------------------------------------
+ * Replaces wx_<algo><A, is_tree>::compute(text, size, levels)
+ * (wx_ppc.hpp:36-58, wx_pps.hpp:37-68, wx_dd_pc.hpp:42-119, wx_pc.hpp,
+ *  wx_ps.hpp, wx_pc_ss.hpp, wx_ppc_ss.hpp, wx_dd_ps.hpp, wx_dd_pc_ss.hpp,
+ *  wx_naive.hpp, wx_huff_pc.hpp:45-80, wx_huff_ps.hpp, wx_huff_pc_ss.hpp,
+ *  wx_huff_naive.hpp, wx_huff_dd_pc.hpp).
+ * word_width = sizeof(A) in {1, 2, 4, 8}.  size == 0 gives an empty
+ * structure (levels = 0), as the reference does. */
+int pwmhip_build(pwmhip_ctx* ctx, const void* text, uint64_t size, uint32_t word_width,
+                 uint32_t levels, uint32_t algo, uint32_t flags, pwmhip_ws* out);
+
+int pwmhip_ws_free(pwmhip_ctx* ctx, pwmhip_ws* ws);
+
+/* Copies level `level` (ceil(level_bits/64) words) to host memory. */
+int pwmhip_ws_download(pwmhip_ctx* ctx, const pwmhip_ws* ws, uint64_t level, uint64_t* host_words);
+
+/* ---- rank / select support ----------------------------------------------
+ * Replaces query_support<> (queries/query_support.hpp:18-186) with its
+ * per-level bin_rank_popcnt (bin_rank_popcnt.hpp:54-86) and
+ * bin_select_popcnt<0/1> (bin_select_popcnt.hpp:38-107) directories.
+ * Tables: per level, 64 Ki-bit superblocks with absolute uint64 counts and
+ * 512-bit blocks with uint16 counts relative to their superblock, plus
+ * select samples every 8192 ones and zeros.  Valid for levels of any size
+ * below 2^63 bits. */
+typedef struct pwmhip_rs pwmhip_rs;
+int pwmhip_rs_build(pwmhip_ctx* ctx, const pwmhip_ws* ws, pwmhip_rs** out);
+int pwmhip_rs_free(pwmhip_ctx* ctx, pwmhip_rs* rs);
+
+/* Bit-vector level queries (bin_rank_popcnt::rank1, bin_select_popcnt::select):
+ * op 0: rank1(level, i)  = ones in [0, i)
+ * op 1: rank0(level, i)
+ * op 2: select1(level, k) = 0-based position of the k-th one (k >= 1; 0 -> 0)
+ * op 3: select0(level, k)
+ * arrays are host or device pointers (flag PWMHIP_FLAG_TEXT_ON_DEVICE). */
+int pwmhip_bv_query(pwmhip_ctx* ctx, const pwmhip_rs* rs, const uint32_t* level,
+                    const uint32_t* op, const uint64_t* arg, uint64_t n, uint64_t* result,
+                    uint32_t flags);
+
+/* Symbol queries over the whole structure (query_support::access / rank /
+ * select, query_support.hpp:37-69), one GPU lane per query:
+ * op 0: access(a)            -> symbol at position a
+ * op 1: rank(symbol a, i b)  -> occurrences of a in [0, b)
+ * op 2: select(symbol a, k b)-> 0-based position of the b-th a (b >= 1),
+ *                               UINT64_MAX if a occurs fewer than b times
+ * Works for matrices and trees, plain and Huffman-shaped. */
+int pwmhip_query(pwmhip_ctx* ctx, const pwmhip_ws* ws, const pwmhip_rs* rs,
+                 const uint32_t* op, const uint64_t* a, const uint64_t* b, uint64_t n,
+                 uint64_t* result, uint32_t flags);
+
+/* ---- bit copy -------------------------------------------------------------
+ * Replaces copy_bits<uint64_t> (construction/merge.hpp:19-190): copies
+ * `len` bits from src[src_off..] to dst[dst_off..] on the device (MSB-first
+ * words); partially covered destination words are OR-ed, covered words are
+ * overwritten.  Pointers are device pointers. */
+int pwmhip_copy_bits(pwmhip_ctx* ctx, uint64_t* d_dst, const uint64_t* d_src, uint64_t dst_off,
+                     uint64_t src_off, uint64_t len);
+
+/* ---- alphabet reduction ----------------------------------------------------
+ * Replaces reduce_alphabet<A> + levels_for_max_char (util/alphabet_util.hpp:
+ * 18-45, 99-106): remaps symbols in first-occurrence order (0, 1, 2, ...),
+ * in place on a device text, and returns the number of levels.  Unlike the
+ * reference's 255-entry table, byte value 255 is handled. */
+int pwmhip_reduce_alphabet(pwmhip_ctx* ctx, void* d_text, uint64_t size, uint32_t word_width,
+                           uint64_t* levels_out);
+
+/* ---- batched construction ------------------------------------------------
+ * No counterpart in the reference (it builds one text per call).  Builds
+ * `count` independent texts (host or device pointers, uniform word width,
+ * levels and algorithm) in one persistent launch per phase from a device
+ * work queue; results identical to one pwmhip_build per text. */
+int pwmhip_build_batch(pwmhip_ctx* ctx, const void* const* texts, const uint64_t* sizes,
+                       uint64_t count, uint32_t word_width, uint32_t levels, uint32_t algo,
+                       uint32_t flags, pwmhip_ws* outs);
+
+/* ---- timing -------------------------------------------------------------
+ * Device time of the last build's kernels (HIP events on the context's
+ * stream), per phase: [0] level-0 + local build, [1] offsets, [2] stitch,
+ * [3] total.  Milliseconds. */
+int pwmhip_last_timing(pwmhip_ctx* ctx, double* ms4);
+
+/* ---- synthetic inputs (the bench's seeded generators, host side) ------- */
+/* kind 0: i.i.d. uniform symbols < 2^levels (splitmix64(seed), LE bytes)
+ * kind 1: DNA-like codes 0..4 (order-2 Markov over ACGT, Dirichlet(1) rows,
+ *         N runs at rate 1e-5 with geometric length, mean 500)
+ * kind 2: Zipf(1.1) over 2^levels ranks through a seeded permutation
+ * Texts are generated in independent 1 Mi-symbol blocks, so the result does
+ * not depend on the thread count. */
+int pwmhip_generate(uint32_t kind, uint64_t seed, uint64_t size, uint32_t word_width,
+                    uint32_t levels, void* host_out);
+
+#ifdef __cplusplus
+}
+#endif
+
+#endif /* PWM_HIP_H */

@@ -1,0 +1,336 @@
+"""pwm_amd -- MI355X (gfx950) wavelet matrix / wavelet tree construction.
+
+Host-side mirror of kurpicz/pwm's construction-algorithm interface
+(include/benchmark/algorithm.hpp:39-191, include/wx_*.hpp) over the C-ABI
+library libpwm_hip.so (include/pwm_hip.h).  Every computation runs in the HIP
+library; if it is missing, importing this package raises.
+
+    import pwm_amd
+    for algo in pwm_amd.algorithm_list():          # algorithm_list<...>
+        ws = algo.compute(text, len(text), levels)  # wx_*::compute
+        ws.levels(), ws.bvs()[l], ws.zeros(), ws.level_bit_size(l)
+"""
+import ctypes
+import os
+
+import numpy as np
+
+_HERE = os.path.dirname(os.path.abspath(__file__))
+LIB_PATH = os.path.join(_HERE, "libpwm_hip.so")
+MAX_LEVELS = 64
+
+OK = 0
+ERR_INVALID = -1
+ERR_NO_DEVICE = -2
+ERR_OOM = -3
+ERR_HIP = -4
+ERR_SYMBOL_RANGE = -5
+ERR_UNSUPPORTED = -6
+ERR_BUDGET = -7
+
+FLAG_TREE = 1
+FLAG_TEXT_ON_DEVICE = 2
+FLAG_CHECK_RANGE = 4
+FLAG_REUSE_OUTPUT = 8
+
+# algorithm ids (pwm_hip.h enum pwmhip_algo)
+ALGOS = {
+    "pc": 0, "ps": 1, "ppc": 2, "pps": 3, "pc_ss": 4, "ppc_ss": 5, "dd_pc": 6, "dd_ps": 7,
+    "dd_pc_ss": 8, "naive": 9, "huff_pc": 10, "huff_ps": 11, "huff_pc_ss": 12, "huff_naive": 13,
+    "huff_dd_pc": 14,
+}
+# reference traits: is_parallel (wx_*.hpp `static constexpr bool is_parallel`)
+_PARALLEL = {"ppc", "pps", "dd_pc", "dd_ps", "dd_pc_ss", "ppc_ss", "huff_dd_pc"}
+
+
+class PwmError(RuntimeError):
+    def __init__(self, status, what=""):
+        self.status = status
+        super().__init__("%s (status %d)%s" % (_status_string(status), status, (": " + what) if what else ""))
+
+
+class _WS(ctypes.Structure):
+    _fields_ = [
+        ("levels", ctypes.c_uint64),
+        ("text_size", ctypes.c_uint64),
+        ("is_tree", ctypes.c_int32),
+        ("is_huffman", ctypes.c_int32),
+        ("d_words", ctypes.c_void_p),
+        ("level_offset", ctypes.c_uint64 * MAX_LEVELS),
+        ("level_bits", ctypes.c_uint64 * MAX_LEVELS),
+        ("n_zeros", ctypes.c_uint64),
+        ("zeros", ctypes.c_uint64 * MAX_LEVELS),
+        ("n_codes", ctypes.c_uint64),
+        ("code_len", ctypes.POINTER(ctypes.c_uint64)),
+        ("code_word", ctypes.POINTER(ctypes.c_uint64)),
+        ("alloc_bytes", ctypes.c_uint64),
+    ]
+
+
+def _load():
+    if not os.path.exists(LIB_PATH):
+        raise ImportError("pwm_amd: %s is missing -- run __graft_entry__.build()" % LIB_PATH)
+    lib = ctypes.CDLL(LIB_PATH)
+    vp, u64, u32 = ctypes.c_void_p, ctypes.c_uint64, ctypes.c_uint32
+    sig = {
+        "pwmhip_create": [ctypes.POINTER(vp), ctypes.c_int],
+        "pwmhip_destroy": [vp],
+        "pwmhip_device_count": [ctypes.POINTER(ctypes.c_int)],
+        "pwmhip_build": [vp, vp, u64, u32, u32, u32, u32, ctypes.POINTER(_WS)],
+        "pwmhip_ws_free": [vp, ctypes.POINTER(_WS)],
+        "pwmhip_ws_download": [vp, ctypes.POINTER(_WS), u64, vp],
+        "pwmhip_rs_build": [vp, ctypes.POINTER(_WS), ctypes.POINTER(vp)],
+        "pwmhip_rs_free": [vp, vp],
+        "pwmhip_bv_query": [vp, vp, vp, vp, vp, u64, vp, u32],
+        "pwmhip_query": [vp, ctypes.POINTER(_WS), vp, vp, vp, vp, u64, vp, u32],
+        "pwmhip_copy_bits": [vp, vp, vp, u64, u64, u64],
+        "pwmhip_reduce_alphabet": [vp, vp, u64, u32, ctypes.POINTER(u64)],
+        "pwmhip_build_batch": [vp, vp, vp, u64, u32, u32, u32, u32, ctypes.POINTER(_WS)],
+        "pwmhip_last_timing": [vp, ctypes.POINTER(ctypes.c_double)],
+        "pwmhip_generate": [u32, u64, u64, u32, u32, vp],
+    }
+    for name, args in sig.items():
+        f = getattr(lib, name)
+        f.argtypes = args
+        f.restype = ctypes.c_int
+    lib.pwmhip_status_string.argtypes = [ctypes.c_int]
+    lib.pwmhip_status_string.restype = ctypes.c_char_p
+    return lib
+
+
+_lib = _load()
+
+
+def lib():
+    return _lib
+
+
+def _status_string(st):
+    return _lib.pwmhip_status_string(st).decode()
+
+
+def _check(st, what=""):
+    if st != OK:
+        raise PwmError(st, what)
+
+
+def device_count():
+    c = ctypes.c_int(0)
+    _lib.pwmhip_device_count(ctypes.byref(c))
+    return c.value
+
+
+def generate(kind, seed, size, word_width=1, levels=8):
+    """Seeded synthetic text (host numpy array): kind 'uniform', 'dna' or 'zipf'."""
+    k = {"uniform": 0, "dna": 1, "zipf": 2}[kind]
+    dt = {1: np.uint8, 2: np.uint16, 4: np.uint32, 8: np.uint64}[word_width]
+    out = np.empty(size, dt)
+    _check(_lib.pwmhip_generate(k, seed, size, word_width, levels, out.ctypes.data_as(ctypes.c_void_p)),
+           "generate")
+    return out
+
+
+class Context:
+    """One HIP device, one stream, one scratch arena (pwmhip_ctx)."""
+
+    def __init__(self, device=0):
+        h = ctypes.c_void_p()
+        _check(_lib.pwmhip_create(ctypes.byref(h), device), "create")
+        self.h = h
+        self.device = device
+
+    def close(self):
+        if self.h:
+            _lib.pwmhip_destroy(self.h)
+            self.h = None
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+    def last_timing(self):
+        arr = (ctypes.c_double * 4)()
+        _check(_lib.pwmhip_last_timing(self.h, arr))
+        return list(arr)
+
+
+_default_ctx = None
+
+
+def default_context():
+    global _default_ctx
+    if _default_ctx is None:
+        _default_ctx = Context(0)
+    return _default_ctx
+
+
+def _text_arg(text):
+    """(pointer, size, word_width, flags) for a numpy array or a device tensor."""
+    if hasattr(text, "data_ptr") and hasattr(text, "is_cuda"):
+        if not text.is_cuda:
+            text = text.numpy()
+        else:
+            return ctypes.c_void_p(text.data_ptr()), text.numel(), text.element_size(), FLAG_TEXT_ON_DEVICE
+    if isinstance(text, (bytes, bytearray, str)):
+        text = np.frombuffer(text.encode("utf-8") if isinstance(text, str) else bytes(text), np.uint8)
+    text = np.ascontiguousarray(text)
+    if text.dtype not in (np.uint8, np.uint16, np.uint32, np.uint64):
+        raise PwmError(ERR_INVALID, "text must hold unsigned 8/16/32/64-bit symbols")
+    return text.ctypes.data_as(ctypes.c_void_p), text.size, text.itemsize, 0, text
+
+
+class WaveletStructure:
+    """Result of a construction (reference: construction/wavelet_structure.hpp:
+    142-199).  Levels stay in device memory; bvs() downloads them on first use."""
+
+    def __init__(self, ctx, ws):
+        self._ctx = ctx
+        self._ws = ws
+        self._bvs = None
+
+    # -- reference accessors ------------------------------------------------
+    def levels(self):
+        return int(self._ws.levels)
+
+    def text_size(self):
+        return int(self._ws.text_size)
+
+    def is_tree(self):
+        return bool(self._ws.is_tree)
+
+    def is_huffman_shaped(self):
+        return bool(self._ws.is_huffman)
+
+    def level_bit_size(self, level):
+        return int(self._ws.level_bits[level])
+
+    def zeros(self):
+        return np.array([self._ws.zeros[i] for i in range(self._ws.n_zeros)], dtype=np.uint64)
+
+    def bvs(self):
+        if self._bvs is None:
+            out = []
+            for l in range(self.levels()):
+                nw = (self.level_bit_size(l) + 63) // 64
+                a = np.zeros(max(1, nw), np.uint64)
+                if nw:
+                    _check(_lib.pwmhip_ws_download(self._ctx.h, ctypes.byref(self._ws), l,
+                                                   a.ctypes.data_as(ctypes.c_void_p)), "download")
+                out.append(a[:nw])
+            self._bvs = out
+        return self._bvs
+
+    def codes(self):
+        """Huffman code (length, word) per symbol 0..max_symbol."""
+        n = int(self._ws.n_codes)
+        if not n:
+            return None
+        ln = np.ctypeslib.as_array(self._ws.code_len, shape=(n,)).copy()
+        wd = np.ctypeslib.as_array(self._ws.code_word, shape=(n,)).copy()
+        return ln, wd
+
+    # -- device side --------------------------------------------------------
+    def device_words(self):
+        return int(self._ws.d_words or 0)
+
+    def level_offset(self, level):
+        return int(self._ws.level_offset[level])
+
+    def free(self):
+        if self._ws is not None and self._ctx.h:
+            _lib.pwmhip_ws_free(self._ctx.h, ctypes.byref(self._ws))
+        self._ws = None
+
+    def __del__(self):
+        try:
+            self.free()
+        except Exception:
+            pass
+
+
+def build(text, levels, algo="ppc", is_tree=False, ctx=None, check_range=False, size=None, reuse=None):
+    """pwmhip_build: construct one structure; `text` is a numpy array (host)
+    or a device tensor (torch on cuda:0).  `reuse` is an earlier
+    WaveletStructure whose device memory is taken over."""
+    ctx = ctx or default_context()
+    arg = _text_arg(text)
+    ptr, n, w, flags = arg[0], arg[1], arg[2], arg[3]
+    keep = arg[4] if len(arg) > 4 else text  # keep host array alive
+    if size is not None:
+        n = size
+    if is_tree:
+        flags |= FLAG_TREE
+    if check_range:
+        flags |= FLAG_CHECK_RANGE
+    ws = _WS()
+    if reuse is not None and reuse._ws is not None:
+        ws = reuse._ws
+        reuse._ws = None
+        flags |= FLAG_REUSE_OUTPUT
+    st = _lib.pwmhip_build(ctx.h, ptr, n, w, levels, ALGOS[algo], flags, ctypes.byref(ws))
+    del keep
+    _check(st, "build %s" % algo)
+    return WaveletStructure(ctx, ws)
+
+
+class ConstructionAlgorithm:
+    """Mirror of construction_algorithm (algorithm.hpp:83-191) for one GPU
+    algorithm; `name` follows the reference's registrations
+    (src/register/*.cpp), e.g. "wm_ppc" / "wt_pps" / "wm_huff_pc"."""
+
+    def __init__(self, name, algo, is_tree, word_width, description):
+        self._name = name
+        self._algo = algo
+        self._tree = is_tree
+        self._width = word_width
+        self._description = description
+
+    def name(self):
+        return self._name
+
+    def description(self):
+        return self._description
+
+    def word_width(self):
+        return self._width
+
+    def is_tree(self):
+        return self._tree
+
+    def is_parallel(self):
+        return self._algo in _PARALLEL
+
+    def is_huffman_shaped(self):
+        return self._algo.startswith("huff")
+
+    def is_input_external(self):
+        return False
+
+    def is_output_external(self):
+        return False
+
+    def compute(self, text, size, levels, ctx=None):
+        return build(text, levels, self._algo, self._tree, ctx=ctx, size=size)
+
+    # compute_bitvector (algorithm.hpp:106-125) returns the structure
+    compute_bitvector = compute
+
+
+def algorithm_list(word_width=1):
+    """Mirror of algorithm_list<false,false,w>::get_algorithm_list()."""
+    out = []
+    for algo in ALGOS:
+        for tree in (False, True):
+            name = ("wt_" if tree else "wm_") + algo
+            out.append(ConstructionAlgorithm(name, algo, tree, word_width,
+                                             "gfx950 HIP construction (%s)" % algo))
+    return out
+
+
+def get_algorithm(name, word_width=1):
+    for a in algorithm_list(word_width):
+        if a.name() == name:
+            return a
+    raise KeyError(name)

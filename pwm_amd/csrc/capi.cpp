@@ -1,0 +1,229 @@
+// pwm_amd/csrc/capi.cpp -- the C-ABI entry points (include/pwm_hip.h).
+//
+// Each entry point translates exceptions into status codes, so nothing
+// throws across the ABI.  The algorithm classes of the reference
+// (include/wx_*.hpp) map onto pwmhip_build with an algorithm id; they differ
+// only in zeros[levels-1] (see pwm_hip.h).
+
+#include <cstdlib>
+#include <cstring>
+#include <new>
+
+#include "pwm_internal.hpp"
+
+namespace pwm {
+
+void arena::ensure(size_t bytes) {
+  if (bytes <= cap) return;
+  release();
+  void* p = nullptr;
+  PWM_HIP_TRY(hipMalloc(&p, bytes));
+  base = p;
+  cap = bytes;
+  used = 0;
+}
+
+void arena::release() {
+  if (base) (void)hipFree(base);
+  base = nullptr;
+  cap = 0;
+  used = 0;
+}
+
+namespace {
+
+int run_guarded(const char* /*what*/, auto&& fn) {
+  try {
+    fn();
+    return PWMHIP_OK;
+  } catch (const hip_failure& f) {
+    (void)hipGetLastError();
+    return f.status;
+  } catch (const status_error& e) {
+    return e.status;
+  } catch (const std::bad_alloc&) {
+    return PWMHIP_ERR_OOM;
+  } catch (...) {
+    return PWMHIP_ERR_HIP;
+  }
+}
+
+// zeros[levels-1] is never computed by these reference algorithms.
+bool zeros_last_is_zero(uint32_t algo) {
+  return algo == PWMHIP_ALGO_PPC || algo == PWMHIP_ALGO_PPC_SS || algo == PWMHIP_ALGO_PC_SS ||
+         algo == PWMHIP_ALGO_DD_PC_SS;
+}
+
+bool is_huffman_algo(uint32_t algo) { return algo >= PWMHIP_ALGO_HUFF_PC && algo <= PWMHIP_ALGO_HUFF_DD_PC; }
+
+}  // namespace
+
+}  // namespace pwm
+
+extern "C" {
+
+const char* pwmhip_status_string(int status) {
+  switch (status) {
+    case PWMHIP_OK: return "ok";
+    case PWMHIP_ERR_INVALID: return "invalid argument";
+    case PWMHIP_ERR_NO_DEVICE: return "no HIP device";
+    case PWMHIP_ERR_OOM: return "out of device memory";
+    case PWMHIP_ERR_HIP: return "HIP runtime error";
+    case PWMHIP_ERR_SYMBOL_RANGE: return "symbol does not fit into the requested levels";
+    case PWMHIP_ERR_UNSUPPORTED: return "unsupported input (undefined in the reference)";
+    case PWMHIP_ERR_BUDGET: return "request exceeds the device budget";
+    default: return "unknown status";
+  }
+}
+
+int pwmhip_device_count(int* count) {
+  if (!count) return PWMHIP_ERR_INVALID;
+  int c = 0;
+  if (hipGetDeviceCount(&c) != hipSuccess) {
+    (void)hipGetLastError();
+    *count = 0;
+    return PWMHIP_ERR_NO_DEVICE;
+  }
+  *count = c;
+  return PWMHIP_OK;
+}
+
+int pwmhip_create(pwmhip_ctx** out, int device) {
+  if (!out) return PWMHIP_ERR_INVALID;
+  *out = nullptr;
+  int c = 0;
+  if (hipGetDeviceCount(&c) != hipSuccess || device < 0 || device >= c) {
+    (void)hipGetLastError();
+    return PWMHIP_ERR_NO_DEVICE;
+  }
+  auto* ctx = new (std::nothrow) pwmhip_ctx();
+  if (!ctx) return PWMHIP_ERR_OOM;
+  int st = pwm::run_guarded("create", [&] {
+    ctx->device = device;
+    PWM_HIP_TRY(hipSetDevice(device));
+    int cus = 0;
+    PWM_HIP_TRY(hipDeviceGetAttribute(&cus, hipDeviceAttributeMultiprocessorCount, device));
+    ctx->num_cus = cus > 0 ? cus : 256;
+    PWM_HIP_TRY(hipStreamCreateWithFlags(&ctx->stream, hipStreamNonBlocking));
+    for (auto& e : ctx->ev) PWM_HIP_TRY(hipEventCreate(&e));
+  });
+  if (st != PWMHIP_OK) {
+    pwmhip_destroy(ctx);
+    return st;
+  }
+  *out = ctx;
+  return PWMHIP_OK;
+}
+
+int pwmhip_destroy(pwmhip_ctx* ctx) {
+  if (!ctx) return PWMHIP_ERR_INVALID;
+  (void)hipSetDevice(ctx->device);
+  if (ctx->stream) (void)hipStreamSynchronize(ctx->stream);
+  ctx->scratch.release();
+  ctx->text_scratch.release();
+  ctx->rs_scratch.release();
+  if (ctx->pinned) (void)hipHostFree(ctx->pinned);
+  for (auto& e : ctx->ev)
+    if (e) (void)hipEventDestroy(e);
+  if (ctx->stream) (void)hipStreamDestroy(ctx->stream);
+  delete ctx;
+  return PWMHIP_OK;
+}
+
+int pwmhip_build(pwmhip_ctx* ctx, const void* text, uint64_t size, uint32_t word_width,
+                 uint32_t levels, uint32_t algo, uint32_t flags, pwmhip_ws* out) {
+  if (!ctx || !out) return PWMHIP_ERR_INVALID;
+  uint64_t* reuse_ptr = nullptr;
+  uint64_t reuse_bytes = 0;
+  if ((flags & PWMHIP_FLAG_REUSE_OUTPUT) && out->d_words) {
+    reuse_ptr = out->d_words;
+    reuse_bytes = out->alloc_bytes;
+    std::free(out->code_len);
+    std::free(out->code_word);
+  }
+  std::memset(out, 0, sizeof(*out));
+  if (word_width != 1 && word_width != 2 && word_width != 4 && word_width != 8) return PWMHIP_ERR_INVALID;
+  if (algo > PWMHIP_ALGO_HUFF_DD_PC) return PWMHIP_ERR_INVALID;
+  const bool huff = pwm::is_huffman_algo(algo);
+  const bool tree = (flags & PWMHIP_FLAG_TREE) != 0;
+  out->is_tree = tree;
+  out->is_huffman = huff;
+  if (!huff && (levels == 0 || levels > 8 * word_width || levels > PWMHIP_MAX_LEVELS)) return PWMHIP_ERR_INVALID;
+  if (size > 0 && !text) return PWMHIP_ERR_INVALID;
+  return pwm::run_guarded("build", [&] {
+    PWM_HIP_TRY(hipSetDevice(ctx->device));
+    const void* d_text = text;
+    if (size > 0 && !(flags & PWMHIP_FLAG_TEXT_ON_DEVICE)) {
+      ctx->text_scratch.ensure(size * word_width + 64);
+      PWM_HIP_TRY(hipMemcpyAsync(ctx->text_scratch.base, text, size * word_width, hipMemcpyHostToDevice,
+                                 ctx->stream));
+      d_text = ctx->text_scratch.base;
+    }
+    out->text_size = size;
+    if (reuse_ptr && (huff || size == 0)) {
+      (void)hipFree(reuse_ptr);
+      reuse_ptr = nullptr;
+    }
+    if (huff) {
+      const bool quirks = algo != PWMHIP_ALGO_HUFF_NAIVE;
+      pwm::build_huffman_device(ctx, d_text, size, word_width, tree, quirks, out);
+      return;
+    }
+    if (size == 0) {  // wx_ppc.hpp:39-45: empty structure
+      out->levels = 0;
+      return;
+    }
+    std::vector<uint64_t> bits(levels, size);
+    pwm::alloc_structure(out, levels, bits.data(), reuse_ptr, reuse_bytes);
+    reuse_ptr = nullptr;
+    try {
+      pwm::build_plain_device(ctx, d_text, size, word_width, levels, tree,
+                              (flags & PWMHIP_FLAG_CHECK_RANGE) != 0, out);
+    } catch (...) {
+      (void)hipFree(out->d_words);
+      out->d_words = nullptr;
+      throw;
+    }
+    if (!tree && pwm::zeros_last_is_zero(algo)) out->zeros[levels - 1] = 0;
+  });
+}
+
+int pwmhip_ws_free(pwmhip_ctx* ctx, pwmhip_ws* ws) {
+  if (!ctx || !ws) return PWMHIP_ERR_INVALID;
+  (void)hipSetDevice(ctx->device);
+  if (ws->d_words) (void)hipFree(ws->d_words);
+  std::free(ws->code_len);
+  std::free(ws->code_word);
+  std::memset(ws, 0, sizeof(*ws));
+  return PWMHIP_OK;
+}
+
+int pwmhip_ws_download(pwmhip_ctx* ctx, const pwmhip_ws* ws, uint64_t level, uint64_t* host_words) {
+  if (!ctx || !ws || level >= ws->levels || !host_words) return PWMHIP_ERR_INVALID;
+  return pwm::run_guarded("download", [&] {
+    PWM_HIP_TRY(hipSetDevice(ctx->device));
+    const uint64_t nw = pwm::words_for(ws->level_bits[level]);
+    if (nw == 0) return;
+    PWM_HIP_TRY(hipMemcpyAsync(host_words, ws->d_words + ws->level_offset[level], nw * 8,
+                               hipMemcpyDeviceToHost, ctx->stream));
+    PWM_HIP_TRY(hipStreamSynchronize(ctx->stream));
+  });
+}
+
+int pwmhip_last_timing(pwmhip_ctx* ctx, double* ms4) {
+  if (!ctx || !ms4) return PWMHIP_ERR_INVALID;
+  for (int i = 0; i < 4; ++i) ms4[i] = ctx->last_ms[i];
+  return PWMHIP_OK;
+}
+
+int pwmhip_copy_bits(pwmhip_ctx* ctx, uint64_t* d_dst, const uint64_t* d_src, uint64_t dst_off,
+                     uint64_t src_off, uint64_t len) {
+  if (!ctx || (len && (!d_dst || !d_src))) return PWMHIP_ERR_INVALID;
+  return pwm::run_guarded("copy_bits", [&] {
+    PWM_HIP_TRY(hipSetDevice(ctx->device));
+    pwm::copy_bits_device(ctx->stream, d_dst, d_src, dst_off, src_off, len);
+    PWM_HIP_TRY(hipStreamSynchronize(ctx->stream));
+  });
+}
+
+}  // extern "C"

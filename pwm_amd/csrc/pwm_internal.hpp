@@ -1,0 +1,113 @@
+// pwm_amd/csrc/pwm_internal.hpp -- shared declarations of the gfx950 engine.
+#pragma once
+
+#include <hip/hip_runtime.h>
+
+#include <cstdint>
+#include <cstdio>
+#include <string>
+#include <vector>
+
+#include "../../include/pwm_hip.h"
+
+namespace pwm {
+
+// ---- status handling ------------------------------------------------------
+struct hip_failure {
+  int status;
+  hipError_t err;
+  const char* where;
+};
+
+#define PWM_HIP_TRY(expr)                                                  \
+  do {                                                                     \
+    hipError_t e_ = (expr);                                                \
+    if (e_ != hipSuccess) {                                                \
+      throw ::pwm::hip_failure{e_ == hipErrorOutOfMemory ? PWMHIP_ERR_OOM \
+                                                         : PWMHIP_ERR_HIP, \
+                               e_, #expr};                                 \
+    }                                                                      \
+  } while (0)
+
+struct status_error {
+  int status;
+  std::string what;
+};
+
+// ---- geometry ---------------------------------------------------------------
+// A chunk is the unit of the per-workgroup local build: CHUNK_WORDS 64-bit
+// words per bit plane, i.e. CHUNK_WORDS*64 symbols.
+constexpr int kChunkWords = 256;
+constexpr uint64_t kChunkSyms = uint64_t(kChunkWords) * 64;
+constexpr int kMaxGroup = 8;  // levels built per pass (2^8 buckets per chunk)
+
+inline uint64_t words_for(uint64_t bits) { return (bits + 63) >> 6; }
+// words between consecutive levels of one allocation: 256-byte aligned with
+// at least two spare words so that funnel-shift reads past the last word of
+// a level stay inside the allocation.
+inline uint64_t level_stride(uint64_t bits) { return (words_for(bits) + 2 + 31) & ~uint64_t(31); }
+
+// ---- device scratch arena (per context) ------------------------------------
+struct arena {
+  void* base = nullptr;
+  size_t cap = 0;
+  size_t used = 0;
+  void reset() { used = 0; }
+  void ensure(size_t bytes);  // reallocates (discarding contents) if needed
+  template <typename T>
+  T* take(size_t count) {
+    size_t bytes = (count * sizeof(T) + 255) & ~size_t(255);
+    if (used + bytes > cap) throw status_error{PWMHIP_ERR_OOM, "arena overflow"};
+    T* p = reinterpret_cast<T*>(static_cast<char*>(base) + used);
+    used += bytes;
+    return p;
+  }
+  void release();
+};
+
+}  // namespace pwm
+
+struct pwmhip_ctx {
+  int device = 0;
+  int num_cus = 256;
+  hipStream_t stream = nullptr;
+  hipEvent_t ev[4] = {nullptr, nullptr, nullptr, nullptr};
+  double last_ms[4] = {0, 0, 0, 0};
+  pwm::arena scratch;       // per-build temporaries
+  pwm::arena text_scratch;  // device copy of a host text
+  pwm::arena rs_scratch;    // query staging
+  // pinned staging for small device->host results
+  uint64_t* pinned = nullptr;
+  size_t pinned_words = 0;
+};
+
+namespace pwm {
+
+// ---- plain (non-Huffman) construction -------------------------------------
+// Builds `levels` levels of a wavelet matrix (tree=false) or tree (tree=true)
+// for a device-resident text into ws (which the caller has sized).  Fills
+// ws->zeros with the true zero counts (matrix).  Throws on error.
+void build_plain_device(pwmhip_ctx* ctx, const void* d_text, uint64_t n, uint32_t width,
+                        uint32_t levels, bool tree, bool check_range, pwmhip_ws* ws);
+
+// Allocates ws->d_words for `levels` levels of `bits[l]` bits each and fills
+// the offsets.
+// An earlier allocation (reuse, reuse_bytes) is kept if it is large enough
+// and freed otherwise.
+void alloc_structure(pwmhip_ws* ws, uint64_t levels, const uint64_t* bits,
+                     uint64_t* reuse = nullptr, uint64_t reuse_bytes = 0);
+
+// Huffman-shaped construction (huffman.hip).
+void build_huffman_device(pwmhip_ctx* ctx, const void* d_text, uint64_t n, uint32_t width,
+                          bool tree, bool pc_quirks, pwmhip_ws* ws);
+
+// copy_bits on the device (bitops.hip).
+void copy_bits_device(hipStream_t s, uint64_t* dst, const uint64_t* src, uint64_t dst_off,
+                      uint64_t src_off, uint64_t len);
+
+// WT levels from WM levels by moving each level's buckets into numeric
+// prefix order (bitops.hip).
+void wm_to_wt_device(pwmhip_ctx* ctx, const void* d_text, uint64_t n, uint32_t width,
+                     uint32_t levels, pwmhip_ws* ws);
+
+}  // namespace pwm

@@ -1,0 +1,44 @@
+"""Loads the committed golden fixtures (tests/golden/*.npz)."""
+import os
+
+import numpy as np
+
+HERE = os.path.join(os.path.dirname(os.path.abspath(__file__)), "golden")
+PLAIN_ALGOS = ["pc", "ps", "ppc", "pps", "pc_ss", "ppc_ss", "dd_pc", "dd_ps", "dd_pc_ss", "naive"]
+HUFF_ALGOS = ["huff_pc", "huff_naive", "huff_ps", "huff_pc_ss"]
+
+
+def plain_cases():
+    z = np.load(os.path.join(HERE, "golden_plain.npz"))
+    keys = sorted({k.rsplit("_text", 1)[0] for k in z.files if k.endswith("_text")})
+    out = []
+    for k in keys:
+        t = z[k + "_text"]
+        L = int(z[k + "_levels"][0])
+        words = z[k + "_words"]
+        W = (t.size + 63) // 64
+        lv = [words[l * W:(l + 1) * W] for l in range(L)] if t.size else []
+        zeros = {a: z[k + "_zeros_" + a] for a in PLAIN_ALGOS}
+        out.append(dict(key=k, text=t, levels=L, tree=k.endswith("_wt"), bvs=lv, zeros=zeros))
+    return out
+
+
+def huff_cases():
+    z = np.load(os.path.join(HERE, "golden_huff.npz"))
+    keys = sorted({k.rsplit("_text", 1)[0] for k in z.files if k.endswith("_text")})
+    out = []
+    for k in keys:
+        t = z[k + "_text"]
+        d = dict(key=k, text=t, tree=k.endswith("_wt"), algos={})
+        for a in HUFF_ALGOS:
+            bits = [int(b) for b in z[k + "_" + a + "_bits"]]
+            words = z[k + "_" + a + "_words"]
+            lv, off = [], 0
+            for b in bits:
+                nw = (b + 63) // 64
+                lv.append(words[off:off + nw])
+                off += nw
+            d["algos"][a] = dict(bvs=lv, bits=bits, zeros=z[k + "_" + a + "_zeros"],
+                                 codes=(z[k + "_" + a + "_codelen"], z[k + "_" + a + "_codeword"]))
+        out.append(d)
+    return out
