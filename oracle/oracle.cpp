@@ -290,8 +290,8 @@ int oracle_huff_naive(const void* text, uint64_t n, uint32_t width, int is_tree,
 //     levels >= 8 another), so unassigned entries keep earlier values;
 //   * zeros[l-1] = borders[1] (matrix), zeros[levels-1] stays 0;
 //   * at level 1 symbols of code length 1 still advance their border.
-// Returns -2 if a write would fall outside its level (the reference's
-// behaviour is undefined there).
+// Returns -2 if an occurrence's position falls outside its level (the
+// reference reads and writes past the level there: undefined behaviour).
 // ---------------------------------------------------------------------------
 int oracle_huff_pc(const void* text, uint64_t n, uint32_t width, int is_tree,
                    uint64_t* out, uint64_t* zeros) {
@@ -334,10 +334,8 @@ int oracle_huff_pc(const void* text, uint64_t n, uint32_t width, int is_tree,
       if (len > l) {
         const uint64_t p = code_prefix(len, r.word[s], l);
         const uint64_t pos = B[p]++;
-        if (code_bit(len, r.word[s], l)) {
-          if (pos >= r.level_sizes[l]) return -2;
-          out[off[l] + (pos >> 6)] |= 1ULL << (63 - (pos & 63));
-        }
+        if (pos >= r.level_sizes[l]) return -2;  // outside the level: undefined
+        if (code_bit(len, r.word[s], l)) out[off[l] + (pos >> 6)] |= 1ULL << (63 - (pos & 63));
       } else if (l == 1 && len == 1) {
         B[code_prefix(len, r.word[s], 1)]++;
       }

@@ -334,3 +334,60 @@ def get_algorithm(name, word_width=1):
         if a.name() == name:
             return a
     raise KeyError(name)
+
+
+class QuerySupport:
+    """Rank / select directories over every level plus batched queries
+    (mirror of query_support<>, queries/query_support.hpp:18-186)."""
+
+    ACCESS, RANK, SELECT = 0, 1, 2
+    RANK1, RANK0, SELECT1, SELECT0 = 0, 1, 2, 3
+
+    def __init__(self, ws):
+        self._ws = ws
+        self._ctx = ws._ctx
+        h = ctypes.c_void_p()
+        _check(_lib.pwmhip_rs_build(self._ctx.h, ctypes.byref(ws._ws), ctypes.byref(h)), "rs_build")
+        self.h = h
+
+    def free(self):
+        if self.h and self._ctx.h:
+            _lib.pwmhip_rs_free(self._ctx.h, self.h)
+        self.h = None
+
+    def __del__(self):
+        try:
+            self.free()
+        except Exception:
+            pass
+
+    def bv_query(self, level, op, arg):
+        level = np.ascontiguousarray(level, np.uint32)
+        op = np.ascontiguousarray(op, np.uint32)
+        arg = np.ascontiguousarray(arg, np.uint64)
+        res = np.zeros(max(1, op.size), np.uint64)
+        vp = ctypes.c_void_p
+        _check(_lib.pwmhip_bv_query(self._ctx.h, self.h, level.ctypes.data_as(vp), op.ctypes.data_as(vp),
+                                    arg.ctypes.data_as(vp), op.size, res.ctypes.data_as(vp), 0), "bv_query")
+        return res[:op.size]
+
+    def query(self, op, a, b=None):
+        """op 0 access(a); op 1 rank(symbol a, prefix b); op 2 select(symbol a, k b)."""
+        op = np.ascontiguousarray(op, np.uint32)
+        a = np.ascontiguousarray(a, np.uint64)
+        b = np.ascontiguousarray(b if b is not None else np.zeros(op.size), np.uint64)
+        res = np.zeros(max(1, op.size), np.uint64)
+        vp = ctypes.c_void_p
+        _check(_lib.pwmhip_query(self._ctx.h, ctypes.byref(self._ws._ws), self.h, op.ctypes.data_as(vp),
+                                 a.ctypes.data_as(vp), b.ctypes.data_as(vp), op.size, res.ctypes.data_as(vp), 0),
+               "query")
+        return res[:op.size]
+
+    def access(self, i):
+        return int(self.query([0], [i])[0])
+
+    def rank(self, symbol, index):
+        return int(self.query([1], [symbol], [index])[0])
+
+    def select(self, symbol, occ):
+        return int(self.query([2], [symbol], [occ])[0])

@@ -44,7 +44,113 @@ __global__ void k_copy_bits(uint64_t* dst, const uint64_t* src, uint64_t dst_off
   else dst[w] |= v;  // at most the first and the last word; distinct threads
 }
 
+template <typename T>
+__global__ void k_leaf_hist(const T* text, uint64_t n, uint32_t L, unsigned long long* hist) {
+  for (uint64_t i = uint64_t(blockIdx.x) * blockDim.x + threadIdx.x; i < n; i += uint64_t(gridDim.x) * blockDim.x) {
+    const uint64_t c = text[i];
+    if (L >= 64 || (c >> L) == 0) atomicAdd(&hist[c], 1ull);
+  }
+}
+
+// One thread per destination word of a tree level; the word gathers the bits
+// of every bucket run that overlaps it (runs sorted by destination).
+__global__ void k_gather_runs(uint64_t* dst, const uint64_t* src, const uint64_t* dstart,
+                              const uint64_t* sstart, uint64_t nruns, uint64_t nbits) {
+  const uint64_t w = uint64_t(blockIdx.x) * blockDim.x + threadIdx.x;
+  const uint64_t nw = (nbits + 63) >> 6;
+  if (w >= nw) return;
+  const uint64_t ws = w << 6, we = min(ws + 64, nbits);
+  // last run with dstart <= ws
+  uint64_t lo = 0, hi = nruns - 1;
+  while (lo < hi) {
+    const uint64_t mid = (lo + hi + 1) >> 1;
+    if (dstart[mid] <= ws) lo = mid;
+    else hi = mid - 1;
+  }
+  uint64_t v = 0;
+  for (uint64_t r = lo; r < nruns && dstart[r] < we; ++r) {
+    const uint64_t d0 = dstart[r], d1 = dstart[r + 1];  // dstart[nruns] = nbits
+    const uint64_t a = max(d0, ws), b = min(d1, we);
+    if (a >= b) continue;
+    const int64_t s = int64_t(sstart[r]) - int64_t(d0) + int64_t(ws);
+    v |= window64(src, s, uint32_t(b - ws)) & range_mask(uint32_t(a - ws), uint32_t(b - ws));
+  }
+  dst[w] = v;
+}
+
 }  // namespace
+
+void wm_to_wt_device(pwmhip_ctx* ctx, const void* d_text, uint64_t n, uint32_t width, uint32_t L,
+                     pwmhip_ws* ws) {
+  if (L > 24) throw status_error{PWMHIP_ERR_UNSUPPORTED, "tree above 24 levels"};
+  hipStream_t st = ctx->stream;
+  const uint64_t K = 1ull << L;
+  ctx->scratch.reset();
+  ctx->scratch.ensure(K * 8 + 4 * (K + 2) * 8 + 4096);
+  unsigned long long* d_hist = ctx->scratch.take<unsigned long long>(K);
+  PWM_HIP_TRY(hipMemsetAsync(d_hist, 0, K * 8, st));
+  const uint32_t grid = uint32_t(std::min<uint64_t>((n + 255) / 256, uint64_t(ctx->num_cus) * 16));
+  switch (width) {
+    case 1: hipLaunchKernelGGL(k_leaf_hist<uint8_t>, dim3(grid), dim3(256), 0, st, (const uint8_t*)d_text, n, L, d_hist); break;
+    case 2: hipLaunchKernelGGL(k_leaf_hist<uint16_t>, dim3(grid), dim3(256), 0, st, (const uint16_t*)d_text, n, L, d_hist); break;
+    case 4: hipLaunchKernelGGL(k_leaf_hist<uint32_t>, dim3(grid), dim3(256), 0, st, (const uint32_t*)d_text, n, L, d_hist); break;
+    default: hipLaunchKernelGGL(k_leaf_hist<uint64_t>, dim3(grid), dim3(256), 0, st, (const uint64_t*)d_text, n, L, d_hist); break;
+  }
+  PWM_HIP_TRY(hipGetLastError());
+  std::vector<uint64_t> cnt(K);
+  PWM_HIP_TRY(hipMemcpyAsync(cnt.data(), d_hist, K * 8, hipMemcpyDeviceToHost, st));
+  PWM_HIP_TRY(hipStreamSynchronize(st));
+  uint64_t* d_dstart = ctx->scratch.take<uint64_t>(K / 2 + 2);
+  uint64_t* d_sstart = ctx->scratch.take<uint64_t>(K / 2 + 2);
+
+  // new allocation in tree order; level 0 is the text order in both shapes
+  pwmhip_ws tw = *ws;
+  std::vector<uint64_t> bits(L, n);
+  alloc_structure(&tw, L, bits.data());
+  PWM_HIP_TRY(hipMemcpyAsync(tw.d_words + tw.level_offset[0], ws->d_words + ws->level_offset[0],
+                             words_for(n) * 8, hipMemcpyDeviceToDevice, st));
+  std::vector<uint64_t> pc(cnt), ds, ss;
+  for (uint32_t l = L - 1; l >= 1; --l) {
+    // pc: counts of every l-bit prefix
+    std::vector<uint64_t> up(1ull << l);
+    for (uint64_t p = 0; p < up.size(); ++p) up[p] = pc[2 * p] + pc[2 * p + 1];
+    pc.swap(up);
+    const uint64_t nb = 1ull << l;
+    std::vector<uint64_t> wm_start(nb);
+    uint64_t acc = 0;
+    for (uint64_t q = 0; q < nb; ++q) {  // matrix position order: prefix reverse_l(q)
+      uint64_t p = 0;
+      for (uint32_t i = 0; i < l; ++i) p |= ((q >> i) & 1) << (l - 1 - i);
+      wm_start[p] = acc;
+      acc += pc[p];
+    }
+    ds.clear();
+    ss.clear();
+    acc = 0;
+    for (uint64_t p = 0; p < nb; ++p) {
+      if (pc[p]) {
+        ds.push_back(acc);
+        ss.push_back(wm_start[p]);
+      }
+      acc += pc[p];
+    }
+    ds.push_back(n);
+    ss.push_back(0);
+    const uint64_t nruns = ds.size() - 1;
+    PWM_HIP_TRY(hipMemcpyAsync(d_dstart, ds.data(), ds.size() * 8, hipMemcpyHostToDevice, st));
+    PWM_HIP_TRY(hipMemcpyAsync(d_sstart, ss.data(), ss.size() * 8, hipMemcpyHostToDevice, st));
+    const uint64_t nw = words_for(n);
+    hipLaunchKernelGGL(k_gather_runs, dim3(uint32_t((nw + 255) / 256)), dim3(256), 0, st,
+                       tw.d_words + tw.level_offset[l], ws->d_words + ws->level_offset[l], d_dstart, d_sstart,
+                       nruns, n);
+    PWM_HIP_TRY(hipGetLastError());
+    PWM_HIP_TRY(hipStreamSynchronize(st));  // host vectors are reused next level
+  }
+  (void)hipFree(ws->d_words);
+  ws->d_words = tw.d_words;
+  ws->alloc_bytes = tw.alloc_bytes;
+  for (uint32_t l = 0; l < L; ++l) ws->level_offset[l] = tw.level_offset[l];
+}
 
 void copy_bits_device(hipStream_t st, uint64_t* dst, const uint64_t* src, uint64_t dst_off,
                       uint64_t src_off, uint64_t len) {

@@ -544,6 +544,16 @@ __global__ __launch_bounds__(256) void k_stitch(StitchParams p) {
   }
 }
 
+}  // namespace
+
+void launch_scan_slots(hipStream_t s, const uint32_t* counts, uint64_t* pref, uint64_t* totals,
+                       uint64_t nchunks, uint32_t nslot) {
+  hipLaunchKernelGGL(k_scan_slots, dim3(nslot), dim3(1024), 0, s, counts, pref, totals, nchunks, nslot);
+  PWM_HIP_TRY(hipGetLastError());
+}
+
+namespace {
+
 template <int NP>
 void launch_k1(pwmhip_ctx* ctx, const K1Params& prm) {
   const int per_cu = (NP <= 4) ? 2 : 1;
@@ -704,7 +714,6 @@ void build_plain_device(pwmhip_ctx* ctx, const void* d_text, uint64_t n, uint32_
     std::swap(S_out, S_next);
   }
   PWM_HIP_TRY(hipEventRecord(ctx->ev[3], s));
-  if (tree && multi) wm_to_wt_device(ctx, d_text, n, width, L, ws);
 
   // results: zeros per level and the symbol OR
   if (ctx->pinned_words < 65) {
@@ -728,6 +737,8 @@ void build_plain_device(pwmhip_ctx* ctx, const void* d_text, uint64_t n, uint32_
   ctx->last_ms[3] = ms;
   if (check_range && L < 64 && (ctx->pinned[64] >> L) != 0)
     throw status_error{PWMHIP_ERR_SYMBOL_RANGE, "symbol >= 2^levels"};
+  // trees above one pass: matrix levels with buckets moved into numeric order
+  if (tree && multi) wm_to_wt_device(ctx, d_text, n, width, L, ws);
   ws->n_zeros = tree ? 0 : L;
   for (uint32_t l = 0; l < L; ++l) ws->zeros[l] = tree ? 0 : ctx->pinned[l];
 }

@@ -1,0 +1,589 @@
+// pwm_amd/csrc/huffman.hip -- Huffman-shaped wavelet matrices and trees
+// (the wx_huff_* algorithms, reference include/wx_huff_pc.hpp:45-80).
+//
+// Level l of a Huffman-shaped structure holds the occurrences whose code is
+// longer than l.  Every occurrence with code length > l belongs to exactly one
+// internal node (its l-bit code prefix); its bit at level l lands at
+//     start_l[prefix] + (earlier occurrences in text order with that prefix)
+// where start_l comes from one of two host-side plans:
+//   * reference huff_pc bookkeeping (huffman/huff_pc.hpp:19-83,
+//     huff_building_blocks.hpp:186-222): borders assigned by key position,
+//     kept across levels, zeros[l-1] = borders[1];
+//   * naive semantics (huffman/huff_naive.hpp:18-126): matrix buckets in
+//     bit-reversed prefix order, tree nodes in numeric order, true zeros.
+// The code table follows canonical_huff_codes (huffman/huff_codes.hpp:
+// 141-285) with the same standard-library algorithms on the same input order,
+// so ties resolve exactly as in the reference.
+//
+// Device work: histogram; per chunk (one wave, 4096 symbols) counts per
+// internal node and chunk-local level bits ordered by node; scan of node
+// counts over chunks; stitch of every (chunk, node) run to its global place.
+
+#include "pwm_internal.hpp"
+
+#include <algorithm>
+#include <cstring>
+#include <functional>
+#include <numeric>
+#include <queue>
+#include <unordered_map>
+
+namespace pwm {
+
+namespace {
+
+constexpr uint32_t kHChunk = 4096;       // symbols per chunk (one wave)
+constexpr uint32_t kHMaxSym = 4096;      // supported alphabet (max symbol + 1)
+constexpr uint32_t kHMaxLevels = 48;
+
+template <typename T>
+__global__ void k_hist(const T* text, uint64_t n, uint32_t nsym, unsigned long long* hist,
+                       unsigned long long* first, unsigned long long* maxsym) {
+  __shared__ unsigned int lh[256];
+  const bool small = nsym <= 256;
+  if (small)
+    for (uint32_t i = threadIdx.x; i < 256; i += blockDim.x) lh[i] = 0;
+  __syncthreads();
+  unsigned long long mx = 0;
+  for (uint64_t i = uint64_t(blockIdx.x) * blockDim.x + threadIdx.x; i < n; i += uint64_t(gridDim.x) * blockDim.x) {
+    const uint64_t c = text[i];
+    mx = c > mx ? c : mx;
+    if (c < nsym) {
+      if (small) atomicAdd(&lh[c], 1u);
+      else atomicAdd(&hist[c], 1ull);
+      if (first) atomicMin(&first[c], (unsigned long long)i);
+    }
+  }
+  for (int d = 32; d >= 1; d >>= 1) {
+    const unsigned long long o = __shfl_xor(mx, d, 64);
+    mx = o > mx ? o : mx;
+  }
+  if ((threadIdx.x & 63) == 0) atomicMax(maxsym, mx);
+  __syncthreads();
+  if (small)
+    for (uint32_t i = threadIdx.x; i < 256; i += blockDim.x)
+      if (lh[i]) atomicAdd(&hist[i], (unsigned long long)lh[i]);
+}
+
+struct HParams {
+  const void* text;
+  uint32_t width;
+  uint64_t n;
+  uint32_t levels;
+  uint32_t nslot;
+  const uint32_t* code_len;    // per symbol
+  const uint64_t* code_word;   // per symbol
+  const uint32_t* slot_of;     // [sym * levels + l] internal node slot (l >= 1)
+  const uint32_t* slot_level;  // [slot] level
+  const uint32_t* level_slot0; // [l] first slot of level l (slots grouped by level)
+  uint64_t* level0;            // final level 0
+  uint64_t* temp;              // chunk-local level l at temp + l*temp_stride
+  uint64_t temp_stride;
+  uint32_t* counts;            // [c * nslot + slot]
+  uint64_t nchunks;
+};
+
+__device__ __forceinline__ uint64_t load_any(const void* t, uint32_t w, uint64_t i) {
+  switch (w) {
+    case 1: return static_cast<const uint8_t*>(t)[i];
+    case 2: return static_cast<const uint16_t*>(t)[i];
+    case 4: return static_cast<const uint32_t*>(t)[i];
+    default: return static_cast<const uint64_t*>(t)[i];
+  }
+}
+
+__device__ __forceinline__ uint64_t lanemask_lt() {
+  const uint32_t lane = threadIdx.x & 63;
+  return lane ? (~0ull >> (64 - lane)) : 0ull;
+}
+
+// One wave per chunk.  Pass A counts occurrences per internal node; pass B
+// places every occurrence's level bit at chunk-local position
+// local_start[node] + rank (text order inside the node).
+__global__ __launch_bounds__(64) void k_huff_local(HParams p) {
+  __shared__ uint32_t cnt[kHMaxSym];
+  __shared__ uint32_t run[kHMaxSym];
+  __shared__ unsigned long long bits[kHMaxLevels][kHChunk / 64];
+  const uint64_t c = blockIdx.x;
+  const uint32_t lane = threadIdx.x;
+  const uint64_t base = c * kHChunk;
+  const uint32_t Ec = uint32_t(min<uint64_t>(kHChunk, p.n - base));
+  for (uint32_t s = lane; s < p.nslot; s += 64) cnt[s] = 0;
+  for (uint32_t l = 0; l < p.levels; ++l)
+    for (uint32_t w = lane; w < kHChunk / 64; w += 64) bits[l][w] = 0;
+  __syncthreads();
+  // pass A: counts
+  for (uint32_t g = 0; g < Ec; g += 64) {
+    const bool valid = g + lane < Ec;
+    const uint64_t sym = valid ? load_any(p.text, p.width, base + g + lane) : 0;
+    const uint32_t len = valid ? p.code_len[sym] : 0;
+    const uint64_t word = valid ? p.code_word[sym] : 0;
+    // level 0 bit, text order -> final level 0
+    const uint64_t b0 = __ballot(valid && ((word >> (len - 1)) & 1));
+    if (lane == 0) p.level0[(base + g) >> 6] = __builtin_bitreverse64(b0);
+    for (uint32_t l = 1; l < p.levels; ++l) {
+      const bool act = valid && len > l;
+      uint64_t pending = __ballot(act);
+      const uint32_t slot = act ? p.slot_of[sym * p.levels + l] : 0xffffffffu;
+      while (pending) {
+        const uint32_t leader = __ffsll((long long)pending) - 1;
+        const uint32_t ls = __shfl(slot, leader, 64);
+        const uint64_t m = __ballot(slot == ls);
+        if (lane == leader) cnt[ls] += __popcll(m);
+        pending &= ~m;
+      }
+    }
+  }
+  __syncthreads();
+  // chunk-local starts per node (slots of one level are contiguous)
+  if (lane == 0) {
+    for (uint32_t l = 1; l < p.levels; ++l) {
+      uint32_t acc = 0;
+      for (uint32_t s = p.level_slot0[l]; s < p.level_slot0[l + 1]; ++s) {
+        run[s] = acc;
+        acc += cnt[s];
+      }
+    }
+  }
+  for (uint32_t s = lane; s < p.nslot; s += 64) p.counts[c * p.nslot + s] = cnt[s];
+  __syncthreads();
+  // pass B: place bits
+  for (uint32_t g = 0; g < Ec; g += 64) {
+    const bool valid = g + lane < Ec;
+    const uint64_t sym = valid ? load_any(p.text, p.width, base + g + lane) : 0;
+    const uint32_t len = valid ? p.code_len[sym] : 0;
+    const uint64_t word = valid ? p.code_word[sym] : 0;
+    for (uint32_t l = 1; l < p.levels; ++l) {
+      const bool act = valid && len > l;
+      uint64_t pending = __ballot(act);
+      const uint32_t slot = act ? p.slot_of[sym * p.levels + l] : 0xffffffffu;
+      uint32_t pos = 0;
+      while (pending) {
+        const uint32_t leader = __ffsll((long long)pending) - 1;
+        const uint32_t ls = __shfl(slot, leader, 64);
+        const uint64_t m = __ballot(slot == ls);
+        const uint32_t r0 = run[ls];  // read by all lanes before the leader updates it
+        if (slot == ls) pos = r0 + __popcll(m & lanemask_lt());
+        __syncthreads();
+        if (lane == leader) run[ls] = r0 + __popcll(m);
+        __syncthreads();
+        pending &= ~m;
+      }
+      if (act && ((word >> (len - 1 - l)) & 1))
+        atomicOr(&bits[l][pos >> 6], 1ull << (pos & 63));
+    }
+  }
+  __syncthreads();
+  for (uint32_t l = 1; l < p.levels; ++l)
+    for (uint32_t w = lane; w < kHChunk / 64; w += 64)
+      p.temp[l * p.temp_stride + c * (kHChunk / 64) + w] = __builtin_bitreverse64(bits[l][w]);
+}
+
+__device__ __forceinline__ uint64_t window64(const uint64_t* a, int64_t s) {
+  const int64_t w = s >> 6;
+  const uint32_t sh = uint32_t(s & 63);
+  const uint64_t w0 = w >= 0 ? a[w] : 0ull;
+  if (sh == 0) return w0;
+  return (w0 << sh) | (a[w + 1] >> (64 - sh));
+}
+
+__device__ __forceinline__ uint64_t range_mask(uint32_t a, uint32_t b) {
+  const uint64_t hi = a == 0 ? ~0ull : (~0ull >> a);
+  const uint64_t lo = b >= 64 ? ~0ull : ~(~0ull >> b);
+  return hi & lo;
+}
+
+struct HStitch {
+  const uint32_t* counts;
+  const uint64_t* pref;
+  const uint64_t* gstart;      // [slot]
+  const uint32_t* slot_level;
+  const uint32_t* level_slot0;
+  const uint64_t* temp;
+  uint64_t temp_stride;
+  uint64_t* const* dst;        // level base pointers (device array)
+  const uint64_t* level_bits;  // [l]
+  uint32_t nslot, levels;
+  int all_atomic;
+};
+
+// One block per chunk, one thread per (run, destination word), runs looped.
+__global__ __launch_bounds__(256) void k_huff_stitch(HStitch p) {
+  const uint64_t c = blockIdx.x;
+  for (uint32_t l = 1; l < p.levels; ++l) {
+    uint64_t loc = 0;
+    for (uint32_t s = p.level_slot0[l]; s < p.level_slot0[l + 1]; ++s) {
+      const uint64_t len = p.counts[c * p.nslot + s];
+      if (!len) continue;
+      const uint64_t src = c * kHChunk + loc;
+      loc += len;
+      const uint64_t dbit = p.gstart[s] + p.pref[c * p.nslot + s];
+      const uint64_t w0 = dbit >> 6, w1 = (dbit + len - 1) >> 6;
+      const uint64_t nbits = p.level_bits[l];
+      uint64_t* dst = p.dst[l];
+      const uint64_t* sa = p.temp + l * p.temp_stride;
+      for (uint64_t w = w0 + threadIdx.x; w <= w1; w += blockDim.x) {
+        const uint64_t ws = w << 6;
+        const uint64_t a = max(ws, dbit), b = min(ws + 64, dbit + len);
+        const uint64_t v = window64(sa, int64_t(src) - int64_t(dbit - ws)) &
+                           range_mask(uint32_t(a - ws), uint32_t(b - ws));
+        const uint64_t wend = min(ws + 64, nbits);
+        if (!p.all_atomic && a == ws && b == wend) dst[w] = v;
+        else if (v) atomicOr(reinterpret_cast<unsigned long long*>(&dst[w]), (unsigned long long)v);
+      }
+    }
+  }
+}
+
+// ---------------------------------------------------------------------------
+// host planning
+// ---------------------------------------------------------------------------
+struct code_table {
+  std::vector<uint64_t> len, word, freq;
+  uint64_t levels = 0;
+  std::vector<uint64_t> level_sizes;
+};
+
+struct heap_item {
+  uint64_t occ;
+  std::vector<uint64_t> syms;
+  bool operator>(const heap_item& o) const { return occ > o.occ; }
+};
+
+// canonical_huff_codes<A, is_tree>::compute_codes (huff_codes.hpp:141-285)
+// for the histogram `entries` given in the reference's histogram order.
+code_table make_codes(const std::vector<std::pair<uint64_t, uint64_t>>& entries, uint64_t max_sym, bool tree) {
+  code_table ct;
+  if (entries.empty()) return ct;
+  ct.len.assign(max_sym + 1, 0);
+  ct.word.assign(max_sym + 1, 0);
+  ct.freq.assign(max_sym + 1, 0);
+  for (auto& e : entries) ct.freq[e.first] = e.second;
+  std::priority_queue<heap_item, std::vector<heap_item>, std::greater<heap_item>> heap;
+  for (auto& e : entries) heap.emplace(heap_item{e.second, std::vector<uint64_t>{e.first}});
+  if (heap.size() == 1) ++ct.len[heap.top().syms.front()];
+  while (heap.size() > 1) {
+    heap_item x = heap.top();
+    heap.pop();
+    heap_item y = heap.top();
+    heap.pop();
+    x.syms.insert(x.syms.end(), y.syms.begin(), y.syms.end());
+    for (uint64_t s : x.syms) ++ct.len[s];
+    heap.emplace(heap_item{x.occ + y.occ, x.syms});
+  }
+  std::vector<uint64_t> ord(ct.len.size());
+  std::iota(ord.begin(), ord.end(), uint64_t(0));
+  std::sort(ord.begin(), ord.end(), [&](uint64_t a, uint64_t b) { return ct.len[a] < ct.len[b]; });
+  ct.levels = ct.len[ord.back()];
+  ct.level_sizes.assign(ct.levels, 0);
+  size_t k = 0;
+  while (k < ord.size() && ct.len[ord[k]] == 0) ++k;
+  if (!tree) {
+    // code words handed out from the back of the bit-reversal-ordered list
+    uint64_t cur_len = 1;
+    std::vector<uint64_t> pool{0, 1};
+    for (; k < ord.size(); ++k) {
+      const uint64_t want = ct.len[ord[k]];
+      while (cur_len < want) {
+        std::vector<uint64_t> grown;
+        grown.reserve(pool.size() * 2);
+        for (uint64_t v : pool) grown.push_back(v << 1);
+        for (uint64_t v : pool) grown.push_back((v << 1) | 1);
+        pool.swap(grown);
+        ++cur_len;
+      }
+      ct.word[ord[k]] = pool.back();
+      pool.pop_back();
+    }
+  } else {
+    // complemented canonical code words
+    uint64_t cw = 0;
+    for (size_t first = k; k < ord.size(); ++k) {
+      if (k > first) cw = (cw + 1) << (ct.len[ord[k]] - ct.len[ord[k - 1]]);
+      const uint64_t l = ct.len[ord[k]];
+      ct.word[ord[k]] = (~cw) & (l >= 64 ? ~0ull : ((1ull << l) - 1));
+    }
+  }
+  for (uint64_t s = 0; s < ct.len.size(); ++s)
+    if (ct.len[s]) ct.level_sizes[ct.len[s] - 1] += ct.freq[s];
+  for (uint64_t i = ct.levels; i-- > 1;) ct.level_sizes[i - 1] += ct.level_sizes[i];
+  return ct;
+}
+
+template <typename A>
+std::vector<std::pair<uint64_t, uint64_t>> map_order(const std::vector<uint64_t>& hist,
+                                                     const std::vector<uint64_t>& first) {
+  // Insert the distinct symbols in first-occurrence order into the same
+  // unordered_map type the reference fills while scanning the text
+  // (util/histogram.hpp:60-75); its iteration order is the entry order.
+  std::vector<uint64_t> syms;
+  for (uint64_t s = 0; s < hist.size(); ++s)
+    if (hist[s]) syms.push_back(s);
+  std::sort(syms.begin(), syms.end(), [&](uint64_t a, uint64_t b) { return first[a] < first[b]; });
+  std::unordered_map<A, uint64_t> m;
+  for (uint64_t s : syms) m.emplace(A(s), hist[s]);
+  std::vector<std::pair<uint64_t, uint64_t>> out;
+  for (const auto& e : m) out.emplace_back(uint64_t(e.first), e.second);
+  return out;
+}
+
+inline uint64_t prefix_of(uint64_t len, uint64_t word, uint64_t l) { return word >> (len - l); }
+inline uint32_t rev(uint64_t x, uint64_t bits) {
+  uint64_t r = 0;
+  for (uint64_t i = 0; i < bits; ++i) r |= ((x >> i) & 1) << (bits - 1 - i);
+  return uint32_t(r);
+}
+
+}  // namespace
+
+void build_huffman_device(pwmhip_ctx* ctx, const void* d_text, uint64_t n, uint32_t width, bool tree,
+                          bool pc_quirks, pwmhip_ws* ws) {
+  hipStream_t st = ctx->stream;
+  ws->is_tree = tree;
+  ws->is_huffman = 1;
+  // 1. histogram, max symbol (and first occurrences for map ordering)
+  const uint32_t nsym_cap = kHMaxSym;
+  ctx->scratch.reset();
+  ctx->scratch.ensure(size_t(nsym_cap) * 16 + 4096);
+  unsigned long long* d_hist = ctx->scratch.take<unsigned long long>(nsym_cap);
+  unsigned long long* d_first = ctx->scratch.take<unsigned long long>(nsym_cap);
+  unsigned long long* d_max = ctx->scratch.take<unsigned long long>(1);
+  PWM_HIP_TRY(hipMemsetAsync(d_hist, 0, nsym_cap * 8, st));
+  PWM_HIP_TRY(hipMemsetAsync(d_first, 0xff, nsym_cap * 8, st));
+  PWM_HIP_TRY(hipMemsetAsync(d_max, 0, 8, st));
+  const uint32_t nsym = width == 1 ? 256 : nsym_cap;
+  const uint32_t grid = uint32_t(std::min<uint64_t>(std::max<uint64_t>((n + 1023) / 1024, 1), uint64_t(ctx->num_cus) * 8));
+  unsigned long long* first_arg = width == 1 ? nullptr : d_first;
+  if (n) {
+    switch (width) {
+      case 1: hipLaunchKernelGGL(k_hist<uint8_t>, dim3(grid), dim3(256), 0, st, (const uint8_t*)d_text, n, nsym, d_hist, first_arg, d_max); break;
+      case 2: hipLaunchKernelGGL(k_hist<uint16_t>, dim3(grid), dim3(256), 0, st, (const uint16_t*)d_text, n, nsym, d_hist, first_arg, d_max); break;
+      case 4: hipLaunchKernelGGL(k_hist<uint32_t>, dim3(grid), dim3(256), 0, st, (const uint32_t*)d_text, n, nsym, d_hist, first_arg, d_max); break;
+      default: hipLaunchKernelGGL(k_hist<uint64_t>, dim3(grid), dim3(256), 0, st, (const uint64_t*)d_text, n, nsym, d_hist, first_arg, d_max); break;
+    }
+    PWM_HIP_TRY(hipGetLastError());
+  }
+  std::vector<uint64_t> hist(nsym), first(nsym, ~0ull);
+  uint64_t maxsym = 0;
+  PWM_HIP_TRY(hipMemcpyAsync(hist.data(), d_hist, nsym * 8, hipMemcpyDeviceToHost, st));
+  if (first_arg) PWM_HIP_TRY(hipMemcpyAsync(first.data(), d_first, nsym * 8, hipMemcpyDeviceToHost, st));
+  PWM_HIP_TRY(hipMemcpyAsync(&maxsym, d_max, 8, hipMemcpyDeviceToHost, st));
+  PWM_HIP_TRY(hipStreamSynchronize(st));
+  if (n && maxsym >= nsym) throw status_error{PWMHIP_ERR_UNSUPPORTED, "Huffman alphabet above 4096 symbols"};
+
+  // 2. codes (histogram entry order as in util/histogram.hpp:39-76)
+  std::vector<std::pair<uint64_t, uint64_t>> entries;
+  if (width == 1) {
+    for (uint64_t s = 0; s < 256; ++s)
+      if (hist[s]) entries.emplace_back(s, hist[s]);
+  } else if (width == 2) {
+    entries = map_order<uint16_t>(hist, first);
+  } else if (width == 4) {
+    entries = map_order<uint32_t>(hist, first);
+  } else {
+    entries = map_order<uint64_t>(hist, first);
+  }
+  code_table ct = make_codes(entries, maxsym, tree);
+  ws->n_codes = ct.len.size();
+  if (ws->n_codes) {
+    ws->code_len = static_cast<uint64_t*>(std::malloc(ws->n_codes * 8));
+    ws->code_word = static_cast<uint64_t*>(std::malloc(ws->n_codes * 8));
+    if (!ws->code_len || !ws->code_word) throw status_error{PWMHIP_ERR_OOM, "codes"};
+    std::memcpy(ws->code_len, ct.len.data(), ws->n_codes * 8);
+    std::memcpy(ws->code_word, ct.word.data(), ws->n_codes * 8);
+  }
+  const uint64_t L = ct.levels;
+  if (n == 0 || L == 0) {
+    ws->levels = 0;
+    return;
+  }
+  if (L > kHMaxLevels) throw status_error{PWMHIP_ERR_UNSUPPORTED, "Huffman code longer than 48 bits"};
+
+  // 3. internal nodes per level and their global starts
+  const uint64_t S = ct.len.size();
+  std::vector<uint32_t> slot_of(S * L, 0xffffffffu);
+  std::vector<uint32_t> slot_level, level_slot0(L + 1, 0);
+  std::vector<uint64_t> slot_prefix, slot_count;
+  std::vector<uint64_t> start;
+  uint64_t zeros[PWMHIP_MAX_LEVELS] = {};
+  // B state of the reference's border arrays (levels < 8 / >= 8)
+  std::unordered_map<uint64_t, uint64_t> bstate[2];
+  bool overlap = false;
+  for (uint64_t l = 1; l < L; ++l) {
+    level_slot0[l] = uint32_t(slot_level.size());
+    // counts per l-bit prefix of occurrences with code length > l (continuing)
+    // and with code length >= l (the reference's level histogram)
+    std::unordered_map<uint64_t, uint64_t> cont, ge, ends1;
+    std::vector<uint64_t> seeded;  // pre-seeded keys (ctx_huffman.hpp:95-101)
+    std::unordered_map<uint64_t, bool> seen;
+    for (uint64_t s = 0; s < S; ++s) {
+      if (ct.len[s] < l) continue;
+      const uint64_t p = prefix_of(ct.len[s], ct.word[s], l);
+      if (!seen.count(p)) {
+        seen[p] = true;
+        seeded.push_back(p);
+      }
+      ge[p] += ct.freq[s];
+      if (ct.len[s] > l) cont[p] += ct.freq[s];
+    }
+    std::vector<uint64_t> keys;
+    for (auto& e : cont)
+      if (e.second) keys.push_back(e.first);
+    // node order: numeric (tree) or bit-reversed (matrix) -- naive semantics
+    if (tree) std::sort(keys.begin(), keys.end());
+    else std::sort(keys.begin(), keys.end(), [&](uint64_t a, uint64_t b) { return rev(a, l) < rev(b, l); });
+    std::unordered_map<uint64_t, uint64_t> naive_start;
+    uint64_t acc = 0;
+    for (uint64_t p : keys) {
+      naive_start[p] = acc;
+      acc += cont[p];
+    }
+    if (pc_quirks) {
+      auto& B = bstate[l < 8 ? 0 : 1];
+      const uint64_t nk = seeded.size();
+      B[0] = 0;
+      for (uint64_t i = 1; i < nk; ++i) {
+        auto it = ge.find(i - 1);
+        B[i] = B[i - 1] + (it == ge.end() ? 0 : it->second);
+      }
+      if (!tree) zeros[l - 1] = B[1];
+    }
+    std::vector<std::pair<uint64_t, uint64_t>> ranges;
+    for (uint64_t p : keys) {
+      const uint32_t slot = uint32_t(slot_level.size());
+      slot_level.push_back(uint32_t(l));
+      slot_prefix.push_back(p);
+      slot_count.push_back(cont[p]);
+      const uint64_t s0 = pc_quirks ? bstate[l < 8 ? 0 : 1][p] : naive_start[p];
+      if (s0 + cont[p] > ct.level_sizes[l])
+        throw status_error{PWMHIP_ERR_UNSUPPORTED, "reference huff_pc writes outside the level here"};
+      start.push_back(s0);
+      ranges.emplace_back(s0, s0 + cont[p]);
+      for (uint64_t s = 0; s < S; ++s)
+        if (ct.len[s] > l && prefix_of(ct.len[s], ct.word[s], l) == p) slot_of[s * L + l] = slot;
+    }
+    std::sort(ranges.begin(), ranges.end());
+    for (size_t i = 1; i < ranges.size(); ++i)
+      if (ranges[i].first < ranges[i - 1].second) overlap = true;
+    if (pc_quirks) {
+      // borders advance for every processed occurrence (huff_pc.hpp:46-58),
+      // at level 1 including those of code length 1
+      auto& B = bstate[l < 8 ? 0 : 1];
+      for (auto& e : cont) B[e.first] += e.second;
+      if (l == 1)
+        for (uint64_t s = 0; s < S; ++s)
+          if (ct.len[s] == 1) B[ct.word[s]] += ct.freq[s];
+    }
+  }
+  level_slot0[L] = uint32_t(slot_level.size());
+  if (L >= 1) level_slot0[0] = 0;
+  const uint32_t nslot = uint32_t(slot_level.size());
+  if (nslot > kHMaxSym) throw status_error{PWMHIP_ERR_UNSUPPORTED, "too many Huffman nodes"};
+
+  // true zeros (naive): zeros of every level from the codes
+  if (!pc_quirks && !tree) {
+    for (uint64_t s = 0; s < S; ++s)
+      for (uint64_t l = 0; l < ct.len[s]; ++l)
+        if (!((ct.word[s] >> (ct.len[s] - 1 - l)) & 1)) zeros[l] += ct.freq[s];
+  }
+
+  // 4. output allocation
+  std::vector<uint64_t> bits(ct.level_sizes.begin(), ct.level_sizes.end());
+  alloc_structure(ws, L, bits.data());
+  ws->n_zeros = tree ? 0 : L;
+  for (uint64_t l = 0; l < L; ++l) ws->zeros[l] = tree ? 0 : zeros[l];
+
+  // 5. device tables and scratch
+  const uint64_t nchunks = (n + kHChunk - 1) / kHChunk;
+  const uint64_t tstride = level_stride(nchunks * kHChunk);
+  size_t need = 0;
+  auto add = [&](size_t b) { need += (b + 255) & ~size_t(255); };
+  add(S * 4); add(S * 8); add(S * L * 4); add(nslot * 4 + 4); add((L + 1) * 4); add(nslot * 8 + 8);
+  add(L * tstride * 8); add(nchunks * std::max<uint32_t>(nslot, 1) * 4); add(nchunks * std::max<uint32_t>(nslot, 1) * 8);
+  add(nslot * 8 + 8); add(L * 8); add(L * 8);
+  ctx->scratch.reset();
+  ctx->scratch.ensure(need);
+  arena& A = ctx->scratch;
+  uint32_t* d_len = A.take<uint32_t>(S);
+  uint64_t* d_word = A.take<uint64_t>(S);
+  uint32_t* d_slot_of = A.take<uint32_t>(S * L);
+  uint32_t* d_slot_level = A.take<uint32_t>(nslot + 1);
+  uint32_t* d_level_slot0 = A.take<uint32_t>(L + 1);
+  uint64_t* d_start = A.take<uint64_t>(nslot + 1);
+  uint64_t* d_temp = A.take<uint64_t>(L * tstride);
+  uint32_t* d_counts = A.take<uint32_t>(nchunks * std::max<uint32_t>(nslot, 1));
+  uint64_t* d_pref = A.take<uint64_t>(nchunks * std::max<uint32_t>(nslot, 1));
+  uint64_t* d_tot = A.take<uint64_t>(nslot + 1);
+  uint64_t** d_dst = A.take<uint64_t*>(L);
+  uint64_t* d_lbits = A.take<uint64_t>(L);
+  std::vector<uint32_t> len32(S);
+  for (uint64_t s = 0; s < S; ++s) len32[s] = uint32_t(ct.len[s]);
+  std::vector<uint64_t*> dst(L);
+  for (uint64_t l = 0; l < L; ++l) dst[l] = ws->d_words + ws->level_offset[l];
+  PWM_HIP_TRY(hipMemcpyAsync(d_len, len32.data(), S * 4, hipMemcpyHostToDevice, st));
+  PWM_HIP_TRY(hipMemcpyAsync(d_word, ct.word.data(), S * 8, hipMemcpyHostToDevice, st));
+  PWM_HIP_TRY(hipMemcpyAsync(d_slot_of, slot_of.data(), S * L * 4, hipMemcpyHostToDevice, st));
+  if (nslot) {
+    PWM_HIP_TRY(hipMemcpyAsync(d_slot_level, slot_level.data(), nslot * 4, hipMemcpyHostToDevice, st));
+    PWM_HIP_TRY(hipMemcpyAsync(d_start, start.data(), nslot * 8, hipMemcpyHostToDevice, st));
+  }
+  PWM_HIP_TRY(hipMemcpyAsync(d_level_slot0, level_slot0.data(), (L + 1) * 4, hipMemcpyHostToDevice, st));
+  PWM_HIP_TRY(hipMemcpyAsync(d_dst, dst.data(), L * sizeof(uint64_t*), hipMemcpyHostToDevice, st));
+  PWM_HIP_TRY(hipMemcpyAsync(d_lbits, bits.data(), L * 8, hipMemcpyHostToDevice, st));
+  // levels >= 1 are assembled from runs; clear them first
+  for (uint64_t l = 1; l < L; ++l)
+    PWM_HIP_TRY(hipMemsetAsync(dst[l], 0, words_for(bits[l]) * 8, st));
+
+  PWM_HIP_TRY(hipEventRecord(ctx->ev[0], st));
+  HParams hp{};
+  hp.text = d_text;
+  hp.width = width;
+  hp.n = n;
+  hp.levels = uint32_t(L);
+  hp.nslot = nslot;
+  hp.code_len = d_len;
+  hp.code_word = d_word;
+  hp.slot_of = d_slot_of;
+  hp.slot_level = d_slot_level;
+  hp.level_slot0 = d_level_slot0;
+  hp.level0 = dst[0];
+  hp.temp = d_temp;
+  hp.temp_stride = tstride;
+  hp.counts = d_counts;
+  hp.nchunks = nchunks;
+  hipLaunchKernelGGL(k_huff_local, dim3(uint32_t(nchunks)), dim3(64), 0, st, hp);
+  PWM_HIP_TRY(hipGetLastError());
+  PWM_HIP_TRY(hipEventRecord(ctx->ev[1], st));
+  if (nslot) {
+    launch_scan_slots(st, d_counts, d_pref, d_tot, nchunks, nslot);
+    PWM_HIP_TRY(hipEventRecord(ctx->ev[2], st));
+    HStitch hs{};
+    hs.counts = d_counts;
+    hs.pref = d_pref;
+    hs.gstart = d_start;
+    hs.slot_level = d_slot_level;
+    hs.level_slot0 = d_level_slot0;
+    hs.temp = d_temp;
+    hs.temp_stride = tstride;
+    hs.dst = d_dst;
+    hs.level_bits = d_lbits;
+    hs.nslot = nslot;
+    hs.levels = uint32_t(L);
+    hs.all_atomic = overlap ? 1 : 0;
+    hipLaunchKernelGGL(k_huff_stitch, dim3(uint32_t(nchunks)), dim3(256), 0, st, hs);
+    PWM_HIP_TRY(hipGetLastError());
+  } else {
+    PWM_HIP_TRY(hipEventRecord(ctx->ev[2], st));
+  }
+  PWM_HIP_TRY(hipEventRecord(ctx->ev[3], st));
+  PWM_HIP_TRY(hipStreamSynchronize(st));
+  float ms = 0;
+  for (int i = 0; i < 3; ++i) {
+    PWM_HIP_TRY(hipEventElapsedTime(&ms, ctx->ev[i], ctx->ev[i + 1]));
+    ctx->last_ms[i] = ms;
+  }
+  PWM_HIP_TRY(hipEventElapsedTime(&ms, ctx->ev[0], ctx->ev[3]));
+  ctx->last_ms[3] = ms;
+}
+
+}  // namespace pwm

@@ -97,6 +97,10 @@ void build_plain_device(pwmhip_ctx* ctx, const void* d_text, uint64_t n, uint32_
 void alloc_structure(pwmhip_ws* ws, uint64_t levels, const uint64_t* bits,
                      uint64_t* reuse = nullptr, uint64_t reuse_bytes = 0);
 
+// Exclusive scan over chunks of counts[c*nslot + slot] (one block per slot).
+void launch_scan_slots(hipStream_t s, const uint32_t* counts, uint64_t* pref, uint64_t* totals,
+                       uint64_t nchunks, uint32_t nslot);
+
 // Huffman-shaped construction (huffman.hip).
 void build_huffman_device(pwmhip_ctx* ctx, const void* d_text, uint64_t n, uint32_t width,
                           bool tree, bool pc_quirks, pwmhip_ws* ws);

@@ -62,8 +62,6 @@ def test_random_vs_oracle(w, L):
         for kind in ("uniform", "skew"):
             t = _random_case(rng, w, L, n, kind)
             for tree in (False, True):
-                if tree and L > 8:
-                    continue
                 o = O.naive(t, L, tree)
                 ws = pwm_amd.build(t, L, "pc", tree, check_range=True)
                 _assert_same(ws, o.levels, (w, L, n, kind, tree))

@@ -1,0 +1,87 @@
+"""GPU Huffman-shaped construction (wx_huff_pc / wx_huff_naive drop-ins)
+against the reference's golden vectors and the oracle, bit-exact: codes, level
+sizes, every level word, zeros (huff_pc's border bookkeeping included)."""
+import numpy as np
+import pytest
+
+import bitutil
+import golden_data
+import oracle_lib as O
+
+pwm_amd = pytest.importorskip("pwm_amd")
+
+
+def _same(ws, levels, bits, key):
+    assert ws.levels() == len(levels), key
+    assert [ws.level_bit_size(l) for l in range(ws.levels())] == list(bits), key
+    for l, (x, y) in enumerate(zip(ws.bvs(), levels)):
+        assert np.array_equal(x, y), (key, l)
+
+
+@pytest.mark.gpu
+def test_huffman_golden():
+    for c in golden_data.huff_cases():
+        t, tree = c["text"], c["tree"]
+        for algo in ("huff_pc", "huff_naive"):
+            g = c["algos"][algo]
+            ws = pwm_amd.build(t, 0, algo, tree)
+            _same(ws, g["bvs"], g["bits"], (c["key"], algo))
+            ln, wd = ws.codes()
+            assert np.array_equal(ln, g["codes"][0]) and np.array_equal(wd, g["codes"][1]), c["key"]
+            assert np.array_equal(ws.zeros(), g["zeros"]), (c["key"], algo, ws.zeros(), g["zeros"])
+            ws.free()
+
+
+@pytest.mark.gpu
+def test_huffman_dna_large_vs_oracle():
+    t = pwm_amd.generate("dna", 0xC2, 3_000_017)
+    for tree in (False, True):
+        for algo, fn in (("huff_pc", O.huff_pc), ("huff_naive", O.huff_naive)):
+            o = fn(t, tree)
+            ws = pwm_amd.build(t, 0, algo, tree)
+            _same(ws, o.levels, o.bits, (algo, tree))
+            assert np.array_equal(ws.zeros(), o.zeros)
+            ws.free()
+
+
+@pytest.mark.gpu
+def test_huffman_naive_decodes_and_wide_symbols():
+    rng = np.random.default_rng(17)
+    for w, sig in ((1, 256), (2, 1000), (4, 3000)):
+        dt = {1: np.uint8, 2: np.uint16, 4: np.uint32}[w]
+        t = rng.choice(sig, 50000, p=rng.dirichlet(np.ones(sig) * 0.2)).astype(dt)
+        for tree in (False, True):
+            ws = pwm_amd.build(t, 0, "huff_naive", tree)
+            o = O.huff_naive(t, tree)
+            _same(ws, o.levels, o.bits, (w, tree))
+            d = bitutil.decode_huff(ws.bvs(), [ws.level_bit_size(l) for l in range(ws.levels())],
+                                    ws.codes(), tree, t.size)
+            assert np.array_equal(d, t.astype(np.int64))
+            ws.free()
+
+
+@pytest.mark.gpu
+def test_huffman_pc_undefined_cases_are_rejected():
+    """Where the reference's huff_pc writes outside a level (undefined
+    behaviour), the drop-in reports it instead of guessing."""
+    rng = np.random.default_rng(7)
+    rejected = 0
+    for _ in range(30):
+        sig = int(rng.integers(20, 256))
+        t = rng.choice(sig, 3000, p=rng.dirichlet(np.ones(sig))).astype(np.uint8)
+        try:
+            O.huff_pc(t, False)
+            defined = True
+        except RuntimeError:
+            defined = False
+        if defined:
+            ws = pwm_amd.build(t, 0, "huff_pc", False)
+            o = O.huff_pc(t, False)
+            _same(ws, o.levels, o.bits, "defined")
+            ws.free()
+        else:
+            with pytest.raises(pwm_amd.PwmError) as e:
+                pwm_amd.build(t, 0, "huff_pc", False)
+            assert e.value.status == pwm_amd.ERR_UNSUPPORTED
+            rejected += 1
+    assert rejected > 0

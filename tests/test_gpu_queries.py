@@ -1,0 +1,106 @@
+"""GPU rank / select directories and batched access / rank / select queries,
+against brute force (rank/select on bit vectors, symbol queries over the text)
+and the reference's own query tests' semantics
+(test/query_support_tests.cpp:20-85, test/bin_rank_select_tests.cpp:39-62)."""
+import numpy as np
+import pytest
+
+import bitutil
+import oracle_lib as O
+
+pwm_amd = pytest.importorskip("pwm_amd")
+
+
+def _bv_brute(bits, ops, args):
+    pre = np.concatenate([[0], np.cumsum(bits)]).astype(np.int64)
+    ones = np.flatnonzero(bits)
+    zeros = np.flatnonzero(bits == 0)
+    out = []
+    for o, x in zip(ops, args):
+        x = int(x)
+        if o == 0:
+            out.append(pre[min(x, bits.size)])
+        elif o == 1:
+            y = min(x, bits.size)
+            out.append(y - pre[y])
+        elif o == 2:
+            out.append(0 if x == 0 else (ones[x - 1] if x <= ones.size else 2**64 - 1))
+        else:
+            out.append(0 if x == 0 else (zeros[x - 1] if x <= zeros.size else 2**64 - 1))
+    return np.array(out, dtype=np.uint64)
+
+
+@pytest.mark.gpu
+@pytest.mark.parametrize("n", [1, 100, 511, 512, 513, 65535, 65536, 65537, 3 * 65536 + 777, 1 << 21])
+def test_bitvector_rank_select(n):
+    rng = np.random.default_rng(44227 + n)
+    t = rng.integers(0, 2, n).astype(np.uint8)  # 1 level: level 0 == the text bits
+    ws = pwm_amd.build(t, 1, "ppc")
+    qs = pwm_amd.QuerySupport(ws)
+    bits = bitutil.unpack(ws.bvs()[0], n)
+    assert np.array_equal(bits, t)
+    q = 4000
+    ops = rng.integers(0, 4, q).astype(np.uint32)
+    args = np.where(ops < 2, rng.integers(0, n + 1, q), rng.integers(0, n + 2, q)).astype(np.uint64)
+    args[:8] = [0, 1, n, n + 1, 0, 1, 2, 3]
+    ops[:8] = [2, 2, 0, 0, 3, 3, 1, 1]
+    res = qs.bv_query(np.zeros(q, np.uint32), ops, args)
+    assert np.array_equal(res, _bv_brute(t, ops, args))
+
+
+@pytest.mark.gpu
+def test_sparse_and_dense_select():
+    rng = np.random.default_rng(3)
+    for p in (0.0005, 0.9995):
+        n = 1 << 20
+        t = (rng.random(n) < p).astype(np.uint8)
+        ws = pwm_amd.build(t, 1, "pc")
+        qs = pwm_amd.QuerySupport(ws)
+        ones = int(t.sum())
+        ops = np.array([2] * 50 + [3] * 50, np.uint32)
+        args = np.concatenate([rng.integers(1, ones + 1, 50), rng.integers(1, n - ones + 1, 50)]).astype(np.uint64)
+        assert np.array_equal(qs.bv_query(np.zeros(100, np.uint32), ops, args), _bv_brute(t, ops, args))
+
+
+@pytest.mark.gpu
+@pytest.mark.parametrize("tree", [False, True])
+@pytest.mark.parametrize("w,L,n", [(1, 8, 5000), (1, 3, 200003), (2, 11, 70000), (4, 20, 300000)])
+def test_symbol_queries(tree, w, L, n):
+    rng = np.random.default_rng(L * 100 + n)
+    dt = {1: np.uint8, 2: np.uint16, 4: np.uint32}[w]
+    k = min(1 << L, 300)
+    t = rng.choice(k, n, p=rng.dirichlet(np.ones(k) * 0.5)).astype(dt)
+    for algo in ("ppc", "pps"):  # ppc leaves zeros[L-1] = 0: queries must not depend on it
+        ws = pwm_amd.build(t, L, algo, tree)
+        qs = pwm_amd.QuerySupport(ws)
+        q = 20000
+        ops = rng.integers(0, 3, q).astype(np.uint32)
+        syms = t[rng.integers(0, n, q)].astype(np.uint64)
+        a = np.where(ops == 0, rng.integers(0, n + 1, q), syms).astype(np.uint64)
+        b = np.where(ops == 1, rng.integers(0, n + 1, q), rng.integers(0, 50, q)).astype(np.uint64)
+        got = qs.query(ops, a, b)
+        exp = O.symbol_queries(t, ops, a, b)
+        bad = np.flatnonzero(got != exp)
+        assert bad.size == 0, (algo, bad[:5], got[bad[:5]], exp[bad[:5]], ops[bad[:5]], a[bad[:5]], b[bad[:5]])
+        qs.free()
+        ws.free()
+
+
+@pytest.mark.gpu
+def test_reference_query_test_all_symbols():
+    """query_support_tests.cpp:20-85: access and rank of all 256 symbols at
+    every position of a short text, matrix and tree."""
+    t = np.frombuffer("ประเทศไทย中华Việt Nam abracadabra".encode("utf-8"), np.uint8).copy()
+    for tree in (False, True):
+        ws = pwm_amd.build(t, 8, "ppc", tree)
+        qs = pwm_amd.QuerySupport(ws)
+        n = t.size
+        acc = qs.query(np.zeros(n, np.uint32), np.arange(n))
+        assert np.array_equal(acc, t.astype(np.uint64))
+        syms = np.repeat(np.arange(256), n + 1)
+        idx = np.tile(np.arange(n + 1), 256)
+        got = qs.query(np.ones(syms.size, np.uint32), syms, idx)
+        pre = np.zeros((256, n + 1), np.int64)
+        for c in range(256):
+            pre[c, 1:] = np.cumsum(t == c)
+        assert np.array_equal(got, pre[syms, idx].astype(np.uint64))
