@@ -15,6 +15,11 @@ import os
 
 import numpy as np
 
+try:  # share one HIP runtime with torch: it must be loaded before libpwm_hip
+    import torch  # noqa: F401
+except ImportError:
+    torch = None
+
 _HERE = os.path.dirname(os.path.abspath(__file__))
 LIB_PATH = os.path.join(_HERE, "libpwm_hip.so")
 MAX_LEVELS = 64

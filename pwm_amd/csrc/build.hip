@@ -619,31 +619,39 @@ void build_plain_device(pwmhip_ctx* ctx, const void* d_text, uint64_t n, uint32_
   const uint32_t gmax = groups[0];
   const uint32_t npay0 = L - gmax;
   const uint32_t nslot_max = (2u << gmax) - 2u;
-  size_t need = 0;
-  auto add = [&](size_t bytes) { need += (bytes + 255) & ~size_t(255); };
-  add(size_t(std::max<uint32_t>(1, gmax - 1)) * stride * 8);  // temp levels
-  add(size_t(npay0) * stride * 8);                // temp payload
-  add(size_t(npay0) * stride * 8);                // S ping
-  add(size_t(npay0 > kMaxGroup ? npay0 - kMaxGroup : 0) * stride * 8);  // S pong
-  add(nchunks * nslot_max * 4);                   // counts
-  add(nchunks * nslot_max * 8);                   // pref
-  add(nchunks * kMaxGroup * 4);                   // ztot
-  add(nslot_max * 8 * 2);                         // totals, gstart
-  add(64 * 8 + 64);                               // zeros, sym_or
-  ctx->scratch.reset();
-  ctx->scratch.ensure(need);
-  arena& A = ctx->scratch;
-  uint64_t* temp_levels = A.take<uint64_t>(size_t(std::max<uint32_t>(1, gmax - 1)) * stride);
-  uint64_t* temp_payload = A.take<uint64_t>(std::max<size_t>(1, size_t(npay0) * stride));
-  uint64_t* S_ping = A.take<uint64_t>(std::max<size_t>(1, size_t(npay0) * stride));
-  uint64_t* S_pong = A.take<uint64_t>(std::max<size_t>(1, size_t(npay0 > kMaxGroup ? npay0 - kMaxGroup : 0) * stride));
-  uint32_t* counts = A.take<uint32_t>(std::max<size_t>(1, nchunks * nslot_max));
-  uint64_t* pref = A.take<uint64_t>(std::max<size_t>(1, nchunks * nslot_max));
-  uint32_t* ztot = A.take<uint32_t>(nchunks * kMaxGroup);
-  uint64_t* totals = A.take<uint64_t>(nslot_max);
-  uint64_t* gstart = A.take<uint64_t>(nslot_max);
-  uint64_t* zeros_dev = A.take<uint64_t>(64);
-  unsigned long long* sym_or = A.take<unsigned long long>(1);
+  // Every scratch buffer is taken twice: once to size the arena, once for real.
+  const size_t sz_temp = size_t(std::max<uint32_t>(1, gmax - 1)) * stride;
+  const size_t sz_pay = std::max<size_t>(1, size_t(npay0) * stride);
+  const size_t sz_pong = std::max<size_t>(1, size_t(npay0 > kMaxGroup ? npay0 - kMaxGroup : 0) * stride);
+  const size_t sz_cnt = std::max<size_t>(1, nchunks * nslot_max);
+  auto carve = [&](arena& A, uint64_t*& temp_levels, uint64_t*& temp_payload, uint64_t*& S_ping,
+                   uint64_t*& S_pong, uint32_t*& counts, uint64_t*& pref, uint32_t*& ztot, uint64_t*& totals,
+                   uint64_t*& gstart, uint64_t*& zeros_dev, unsigned long long*& sym_or) {
+    temp_levels = A.take<uint64_t>(sz_temp);
+    temp_payload = A.take<uint64_t>(sz_pay);
+    S_ping = A.take<uint64_t>(sz_pay);
+    S_pong = A.take<uint64_t>(sz_pong);
+    counts = A.take<uint32_t>(sz_cnt);
+    pref = A.take<uint64_t>(sz_cnt);
+    ztot = A.take<uint32_t>(nchunks * kMaxGroup);
+    totals = A.take<uint64_t>(nslot_max);
+    gstart = A.take<uint64_t>(nslot_max);
+    zeros_dev = A.take<uint64_t>(64);
+    sym_or = A.take<unsigned long long>(1);
+  };
+  uint64_t *temp_levels, *temp_payload, *S_ping, *S_pong, *pref, *totals, *gstart, *zeros_dev;
+  uint32_t *counts, *ztot;
+  unsigned long long* sym_or;
+  {
+    arena sizing;  // dry run on an unbounded arena to measure
+    sizing.cap = ~size_t(0) >> 1;
+    carve(sizing, temp_levels, temp_payload, S_ping, S_pong, counts, pref, ztot, totals, gstart, zeros_dev,
+          sym_or);
+    ctx->scratch.reset();
+    ctx->scratch.ensure(sizing.used);
+  }
+  carve(ctx->scratch, temp_levels, temp_payload, S_ping, S_pong, counts, pref, ztot, totals, gstart, zeros_dev,
+        sym_or);
   PWM_HIP_TRY(hipMemsetAsync(sym_or, 0, 8, s));
 
   PWM_HIP_TRY(hipEventRecord(ctx->ev[0], s));
