@@ -21,14 +21,14 @@
 //      and the parts are concatenated in LDS at offsets from a block scan.
 //      The chunk-local level bits, the chunk's bucket counts and, for the
 //      next pass, the partitioned remaining planes go to scratch.
-//  K2  k_scan_slots / k_bucket_starts / k_zero_edges: per (level, bucket)
+//  K2  k_tile_* / k_bucket_starts / k_make_runs: per (level, bucket)
 //      exclusive scan over the chunks, global bucket starts (matrix: bucket
 //      index order = bit-reversed prefix, permutation.hpp:36-51; tree:
-//      numeric prefix order), and zeroing of destination words that several
-//      runs share.
-//  K3  k_stitch: every (chunk, level, bucket) run of bits is copied to its
-//      global position (the copy_bits analogue, merge.hpp:19-190); covered
-//      words are stored, shared edge words are OR-ed atomically.
+//      numeric prefix order), and a compact table of the non-empty
+//      (chunk, bucket) runs of every level in destination order.
+//  K3  k_gather: every destination word is assembled once from the runs that
+//      overlap it (the copy_bits analogue, merge.hpp:19-190), with no atomics
+//      and no pre-clearing.
 //
 // More than kMaxGroup levels are built in passes of up to kMaxGroup levels;
 // the partitioned remaining planes of one pass are the next pass's input.
@@ -87,6 +87,14 @@ __device__ __forceinline__ void or_bits(uint64_t* arr, uint32_t pos, uint64_t bi
     atomicOr(reinterpret_cast<unsigned long long*>(&arr[w + 1]), (unsigned long long)(bits >> (64 - sh)));
 }
 
+// Workgroup barrier that orders LDS only: outstanding global loads (the next
+// chunk's prefetch) stay in flight across it.
+__device__ __forceinline__ void lds_sync() {
+  __builtin_amdgcn_fence(__ATOMIC_RELEASE, "workgroup", "local");
+  __builtin_amdgcn_s_barrier();
+  __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "workgroup", "local");
+}
+
 // Exclusive block scan of two counters; totals returned to every thread.
 // sm needs 2 * NT/64 words.  Contains two barriers.
 template <int NT>
@@ -107,7 +115,7 @@ __device__ __forceinline__ void block_exscan2(uint32_t a, uint32_t b, uint32_t& 
     sm[wid] = ia;
     sm[NT / 64 + wid] = ib;
   }
-  __syncthreads();
+  lds_sync();
   uint32_t oa = 0, ob = 0, sa = 0, sb = 0;
 #pragma unroll
   for (int w = 0; w < NT / 64; ++w) {
@@ -123,7 +131,7 @@ __device__ __forceinline__ void block_exscan2(uint32_t a, uint32_t b, uint32_t& 
   eb = ob + ib - b;
   ta = sa;
   tb = sb;
-  __syncthreads();
+  lds_sync();
 }
 
 // rank1 in an LDS plane with per-word exclusive one-counts `ooff`.
@@ -151,14 +159,16 @@ struct K1Params {
   uint64_t temp_stride;
   uint32_t* counts;          // counts[c*nslot + slot]
   uint32_t nslot;
-  uint32_t* ztot;            // ztot[c*g + j]
+  unsigned long long* zeros_out;  // zeros of sub-level j accumulate into zeros_out[j]
   unsigned long long* sym_or;
   uint64_t nchunks;
+  int aligned16;             // byte text 16-byte aligned: vector loads + prefetch
 };
 
 // Gather bit s (0 = least significant) of 4 packed bytes into 4 bits.
+// (v_dot4_u32_u8 multiplies the masked bytes by their place values 1,2,4,8.)
 __device__ __forceinline__ uint32_t gather4(uint32_t x4, uint32_t s) {
-  return (((x4 >> s) & 0x01010101u) * 0x10204080u) >> 28;
+  return __builtin_amdgcn_udot4((x4 >> s) & 0x01010101u, 0x08040201u, 0u, false);
 }
 
 template <int NP, typename T>
@@ -199,36 +209,142 @@ __device__ __forceinline__ void load_planes_text(const K1Params& p, uint64_t e0,
   }
 }
 
+// 64 bytes (one thread's symbols) -> bit planes, symbol bit L-1-l0-k -> plane k.
 template <int NP>
-__global__ __launch_bounds__(P) void k_local_build(K1Params p) {
+__device__ __forceinline__ void planes_from_bytes(const K1Params& p, const uint4* d, uint64_t* acc,
+                                                  uint64_t& my_or) {
+  uint32_t x[16];
+#pragma unroll
+  for (int i = 0; i < 4; ++i) {
+    x[4 * i] = d[i].x;
+    x[4 * i + 1] = d[i].y;
+    x[4 * i + 2] = d[i].z;
+    x[4 * i + 3] = d[i].w;
+  }
+  uint32_t o = 0;
+#pragma unroll
+  for (int q = 0; q < 16; ++q) o |= x[q];
+  my_or |= (o | (o >> 8) | (o >> 16) | (o >> 24)) & 0xffu;
+#pragma unroll
+  for (int k = 0; k < NP; ++k) {
+    if (k < int(p.np)) {
+      // 8 symbols -> one byte of the plane: masked bit s of each byte times
+      // its place value, two 4-byte dot products chained through the
+      // accumulator; the result is scaled by 2^s.
+      const uint32_t sft = p.L - 1 - p.l0 - k;
+      const uint32_t m = 0x01010101u << sft;
+      uint32_t b[8];
+#pragma unroll
+      for (int q = 0; q < 8; ++q)
+        b[q] = __builtin_amdgcn_udot4(x[2 * q + 1] & m, 0x80402010u,
+                                      __builtin_amdgcn_udot4(x[2 * q] & m, 0x08040201u, 0u, false), false) >>
+               sft;
+      const uint32_t lo = b[0] | (b[1] << 8) | (b[2] << 16) | (b[3] << 24);
+      const uint32_t hi = b[4] | (b[5] << 8) | (b[6] << 16) | (b[7] << 24);
+      acc[k] = (uint64_t(hi) << 32) | lo;
+    }
+  }
+}
+
+// Inclusive wave64 prefix sum with DPP row shifts and row broadcasts.
+__device__ __forceinline__ uint32_t wave_incl_scan(uint32_t v) {
+  v += __builtin_amdgcn_update_dpp(0u, v, 0x111, 0xf, 0xf, false);  // row_shr:1
+  v += __builtin_amdgcn_update_dpp(0u, v, 0x112, 0xf, 0xf, false);  // row_shr:2
+  v += __builtin_amdgcn_update_dpp(0u, v, 0x114, 0xf, 0xf, false);  // row_shr:4
+  v += __builtin_amdgcn_update_dpp(0u, v, 0x118, 0xf, 0xf, false);  // row_shr:8
+  v += __builtin_amdgcn_update_dpp(0u, v, 0x142, 0xa, 0xf, false);  // row_bcast:15
+  v += __builtin_amdgcn_update_dpp(0u, v, 0x143, 0xc, 0xf, false);  // row_bcast:31
+  return v;
+}
+
+// Orders this wave's LDS accesses (lanes exchange data through LDS).
+__device__ __forceinline__ void wave_sync() {
+  __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront", "local");
+  __builtin_amdgcn_wave_barrier();
+  __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "wavefront", "local");
+}
+
+// Exclusive wave scan of two counters plus their totals.
+__device__ __forceinline__ void wave_exscan2(uint32_t a, uint32_t b, uint32_t& ea, uint32_t& eb,
+                                             uint32_t& ta, uint32_t& tb) {
+  const uint32_t ia = wave_incl_scan(a), ib = wave_incl_scan(b);
+  ta = __builtin_amdgcn_readlane(ia, 63);
+  tb = __builtin_amdgcn_readlane(ib, 63);
+  ea = ia - a;
+  eb = ib - b;
+}
+
+template <int NP>
+struct WaveLds {
+  static constexpr int kB = (1 << (NP < kMaxGroup ? NP : kMaxGroup)) + 1;
+  uint64_t buf[2][NP][64];
+  uint32_t ooff[65];
+  uint32_t bst[2][kB];
+};
+
+template <int NP>
+constexpr int waves_for() {
+  return NP <= 4 ? 16 : (NP <= 8 ? 8 : (NP <= 12 ? 6 : 4));
+}
+
+// One wave builds one chunk (64 words per plane, one word per lane); the
+// workgroup's waves share the compress table and never synchronise with
+// each other after it is filled.
+template <int NP>
+__global__ __launch_bounds__(64 * waves_for<NP>()) void k_local_build(K1Params p) {
+  constexpr int WAVES = waves_for<NP>();
   __shared__ uint8_t lut[65536];
-  __shared__ uint64_t buf[2][NP][P];
-  __shared__ uint32_t ooff[P + 1];
-  __shared__ uint32_t bst[2][(1 << kMaxGroup) + 1];
-  __shared__ uint32_t scr[2 * P / 64];
+  __shared__ WaveLds<NP> wl[WAVES];
 
-  const int t = threadIdx.x;
-  for (int i = t; i < 65536; i += P) lut[i] = uint8_t(compress8(uint32_t(i) & 255u, uint32_t(i) >> 8));
+  const int lane = threadIdx.x & 63;
+  const int wid = threadIdx.x >> 6;
+  for (int i = threadIdx.x; i < 65536; i += 64 * WAVES)
+    lut[i] = uint8_t(compress8(uint32_t(i) & 255u, uint32_t(i) >> 8));
+  __syncthreads();
+  WaveLds<NP>& L = wl[wid];
   uint64_t my_or = 0;
+  uint64_t zsum[kMaxGroup];  // zeros per sub-level over this wave's chunks
+#pragma unroll
+  for (int j = 0; j < kMaxGroup; ++j) zsum[j] = 0;
 
-  for (uint64_t c = blockIdx.x; c < p.nchunks; c += gridDim.x) {
-    __syncthreads();
+  const uint64_t gw = uint64_t(blockIdx.x) * WAVES + wid;
+  const uint64_t tw = uint64_t(gridDim.x) * WAVES;
+  const bool fast8 = !p.planes_in && p.width == 1 && p.aligned16;
+  auto full_chunk = [&](uint64_t cc) { return (cc + 1) * kChunkSyms <= p.n; };
+  auto load_chunk = [&](uint64_t cc, uint4* d) {
+    const uint4* src = reinterpret_cast<const uint4*>(static_cast<const uint8_t*>(p.text) +
+                                                      cc * kChunkSyms + 64ull * lane);
+    d[0] = src[0];
+    d[1] = src[1];
+    d[2] = src[2];
+    d[3] = src[3];
+  };
+  uint4 pf[4];
+  if (fast8 && gw < p.nchunks && full_chunk(gw)) load_chunk(gw, pf);
+
+  for (uint64_t c = gw; c < p.nchunks; c += tw) {
+    wave_sync();
+    const bool have_cur = fast8 && full_chunk(c);
     const uint64_t base = c * kChunkSyms;
     const uint32_t Ec = uint32_t(min<uint64_t>(kChunkSyms, p.n - base));
     const uint32_t nw = (Ec + 63) >> 6;
-    const int cnt = max(0, min(64, int(Ec) - 64 * t));
+    const int cnt = max(0, min(64, int(Ec) - 64 * lane));
 
     uint64_t acc[NP];
 #pragma unroll
     for (int k = 0; k < NP; ++k) acc[k] = 0;
     if (p.planes_in) {
-      if (t < int(nw)) {
+      if (lane < int(nw)) {
 #pragma unroll
         for (int k = 0; k < NP; ++k)
-          if (k < int(p.np)) acc[k] = brev64(p.planes_in[k * p.in_stride + c * P + t]);
+          if (k < int(p.np)) acc[k] = brev64(p.planes_in[k * p.in_stride + c * P + lane]);
       }
+    } else if (have_cur) {
+      planes_from_bytes<NP>(p, pf, acc, my_or);
+      const uint64_t cn = c + tw;  // prefetch the next chunk of this wave
+      if (cn < p.nchunks && full_chunk(cn)) load_chunk(cn, pf);
     } else if (cnt > 0) {
-      const uint64_t e0 = base + 64ull * t;
+      const uint64_t e0 = base + 64ull * lane;
       switch (p.width) {
         case 1: load_planes_text<NP, uint8_t>(p, e0, cnt, acc, my_or); break;
         case 2: load_planes_text<NP, uint16_t>(p, e0, cnt, acc, my_or); break;
@@ -238,92 +354,126 @@ __global__ __launch_bounds__(P) void k_local_build(K1Params p) {
     }
 #pragma unroll
     for (int k = 0; k < NP; ++k)
-      if (k < int(p.np)) buf[0][k][t] = acc[k];
+      if (k < int(p.np)) L.buf[0][k][lane] = acc[k];
     const uint64_t vm = cnt >= 64 ? ~0ull : (cnt <= 0 ? 0ull : ((1ull << cnt) - 1));
-    if (t == 0) {
-      bst[0][0] = 0;
-      bst[0][1] = Ec;
+    if (lane == 0) {
+      L.bst[0][0] = 0;
+      L.bst[0][1] = Ec;
     }
-    __syncthreads();
+    wave_sync();
 
     int cur = 0;
     uint32_t nb = 1;
     for (uint32_t j = 0; j < p.g; ++j) {
-      const uint64_t m = buf[cur][j][t];
-      if (t < int(nw)) {
-        if (j == 0) p.out_level0[c * P + t] = brev64(m);
-        else p.temp_levels[(j - 1) * p.temp_stride + c * P + t] = brev64(m);
+      const uint64_t m = L.buf[cur][j][lane];
+      if (lane < int(nw)) {
+        if (j == 0) p.out_level0[c * P + lane] = brev64(m);
+        else p.temp_levels[(j - 1) * p.temp_stride + c * P + lane] = brev64(m);
       }
       const uint64_t m1 = m & vm, m0 = ~m & vm;
       const uint32_t o = __popcll(m1), z = __popcll(m0);
-      uint32_t zoff, oof, Zt, Ot;
-      block_exscan2<P>(z, o, zoff, oof, Zt, Ot, scr);
-      if (t == 0) p.ztot[c * p.g + j] = Zt;
+      const uint32_t zinc = wave_incl_scan(z);
+      const uint32_t zoff = zinc - z, Zt = __builtin_amdgcn_readlane(zinc, 63);
+      const uint32_t oof = min(64u * lane, Ec) - zoff;  // every earlier word is full
+      const uint32_t Ot = Ec - Zt;
+      zsum[j] += Zt;
       const bool more = (j + 1 < p.g) || (p.np > p.g);
       if (!more) break;
-      ooff[t] = oof;
-      if (t == P - 1) ooff[P] = Ot;
+      L.ooff[lane] = oof;
+      if (lane == 63) L.ooff[64] = Ot;
       const int nxt = cur ^ 1;
 #pragma unroll
       for (int k = 1; k < NP; ++k)
-        if (k > int(j) && k < int(p.np)) buf[nxt][k][t] = 0;
-      __syncthreads();
+        if (k > int(j) && k < int(p.np)) L.buf[nxt][k][lane] = 0;
+      wave_sync();
 #pragma unroll
       for (int k = 1; k < NP; ++k) {
         if (k > int(j) && k < int(p.np)) {
-          const uint64_t x = buf[cur][k][t];
-          or_bits(buf[nxt][k], zoff, compress64(x, m0, lut), z);
-          or_bits(buf[nxt][k], Zt + oof, compress64(x, m1, lut), o);
+          const uint64_t x = L.buf[cur][k][lane];
+          or_bits(L.buf[nxt][k], zoff, compress64(x, m0, lut), z);
+          or_bits(L.buf[nxt][k], Zt + oof, compress64(x, m1, lut), o);
         }
       }
       // bucket counts of sub-level j+1: zeros and ones of plane j inside
       // every current bucket; new order = zeros of all buckets, then ones.
-      uint32_t zc = 0, oc = 0;
-      if (t < int(nb)) {
-        const uint32_t s = bst[cur][t], e = bst[cur][t + 1];
-        oc = rank1_lds(buf[cur][j], ooff, e) - rank1_lds(buf[cur][j], ooff, s);
-        zc = (e - s) - oc;
-        const uint32_t slot = (2u << j) - 2u;
-        p.counts[c * p.nslot + slot + t] = zc;
-        p.counts[c * p.nslot + slot + nb + t] = oc;
+      const uint32_t slot = (2u << j) - 2u;
+      uint32_t zcar = 0, ocar = 0;
+      for (uint32_t b0 = 0; b0 < nb; b0 += 64) {
+        const uint32_t bi = b0 + lane;
+        uint32_t zc = 0, oc = 0;
+        if (bi < nb) {
+          const uint32_t s0 = L.bst[cur][bi], e0 = L.bst[cur][bi + 1];
+          oc = rank1_lds(L.buf[cur][j], L.ooff, e0) - rank1_lds(L.buf[cur][j], L.ooff, s0);
+          zc = (e0 - s0) - oc;
+          p.counts[c * p.nslot + slot + bi] = zc;
+          p.counts[c * p.nslot + slot + nb + bi] = oc;
+        }
+        uint32_t ez, eo, tz, to;
+        wave_exscan2(zc, oc, ez, eo, tz, to);
+        if (bi < nb) {
+          L.bst[nxt][bi] = zcar + ez;
+          L.bst[nxt][nb + bi] = Zt + ocar + eo;
+        }
+        zcar += tz;
+        ocar += to;
       }
-      uint32_t ez, eo, tz, to;
-      block_exscan2<P>(zc, oc, ez, eo, tz, to, scr);
-      if (t < int(nb)) {
-        bst[nxt][t] = ez;
-        bst[nxt][nb + t] = tz + eo;
-      }
-      if (t == 0) bst[nxt][2 * nb] = Ec;
-      __syncthreads();
+      if (lane == 0) L.bst[nxt][2 * nb] = Ec;
+      wave_sync();
       cur = nxt;
       nb <<= 1;
     }
-    if (t < int(nw)) {
+    if (lane < int(nw)) {
 #pragma unroll
       for (int k = 0; k < NP; ++k)
         if (k >= int(p.g) && k < int(p.np))
-          p.temp_payload[(k - p.g) * p.temp_stride + c * P + t] = brev64(buf[cur][k][t]);
+          p.temp_payload[(k - p.g) * p.temp_stride + c * P + lane] = brev64(L.buf[cur][k][lane]);
     }
+  }
+  if (lane == 0) {
+#pragma unroll
+    for (int j = 0; j < kMaxGroup; ++j)
+      if (j < int(p.g) && zsum[j]) atomicAdd(p.zeros_out + j, (unsigned long long)zsum[j]);
   }
   if (!p.planes_in) {
 #pragma unroll
     for (int d = 32; d >= 1; d >>= 1) my_or |= __shfl_xor(my_or, d, 64);
-    if ((t & 63) == 0 && my_or) atomicOr(p.sym_or, (unsigned long long)my_or);
+    if (lane == 0 && my_or) atomicOr(p.sym_or, (unsigned long long)my_or);
   }
 }
 
 // ---- K2 --------------------------------------------------------------------
-// Exclusive scan over chunks of every slot's count; one block per slot.
-__global__ __launch_bounds__(1024) void k_scan_slots(const uint32_t* counts, uint64_t* pref,
-                                                     uint64_t* totals, uint64_t nchunks,
-                                                     uint32_t nslot) {
+// Parallel exclusive scan over chunks, in tiles of 1024 chunks:
+// k_tile_sums (grid tiles x slots) -> k_tile_scan (grid slots) ->
+// k_tile_apply (grid tiles x slots).
+constexpr uint32_t kTile = 1024;
+
+template <bool IND>
+__global__ __launch_bounds__(256) void k_tile_sums(const uint32_t* counts, uint64_t* tsum,
+                                                   uint64_t nchunks, uint32_t nslot, uint64_t ntiles) {
+  __shared__ uint64_t ws[4];
+  const uint64_t tile = blockIdx.x;
+  const uint32_t slot = blockIdx.y;
+  uint64_t v = 0;
+  for (uint32_t k = threadIdx.x; k < kTile; k += 256) {
+    const uint64_t c = tile * kTile + k;
+    if (c < nchunks) v += IND ? (counts[c * nslot + slot] != 0) : counts[c * nslot + slot];
+  }
+#pragma unroll
+  for (int d = 32; d >= 1; d >>= 1) v += __shfl_xor(v, d, 64);
+  if ((threadIdx.x & 63) == 0) ws[threadIdx.x >> 6] = v;
+  __syncthreads();
+  if (threadIdx.x == 0) tsum[uint64_t(slot) * ntiles + tile] = ws[0] + ws[1] + ws[2] + ws[3];
+}
+
+__global__ __launch_bounds__(1024) void k_tile_scan(uint64_t* tsum, uint64_t* totals, uint64_t ntiles) {
   __shared__ uint64_t wsum[16];
   const uint32_t slot = blockIdx.x;
+  uint64_t* a = tsum + uint64_t(slot) * ntiles;
   const int t = threadIdx.x, lane = t & 63, wid = t >> 6;
   uint64_t carry = 0;
-  for (uint64_t b = 0; b < nchunks; b += 1024) {
-    const uint64_t c = b + t;
-    const uint64_t v = c < nchunks ? counts[c * nslot + slot] : 0;
+  for (uint64_t b = 0; b < ntiles; b += 1024) {
+    const uint64_t i = b + t;
+    const uint64_t v = i < ntiles ? a[i] : 0;
     uint64_t inc = v;
 #pragma unroll
     for (int d = 1; d < 64; d <<= 1) {
@@ -335,33 +485,57 @@ __global__ __launch_bounds__(1024) void k_scan_slots(const uint32_t* counts, uin
     uint64_t off = 0, tot = 0;
 #pragma unroll
     for (int w = 0; w < 16; ++w) {
-      const uint64_t x = wsum[w];
-      if (w < wid) off += x;
-      tot += x;
+      if (w < wid) off += wsum[w];
+      tot += wsum[w];
     }
-    if (c < nchunks) pref[c * nslot + slot] = carry + off + inc - v;
+    if (i < ntiles) a[i] = carry + off + inc - v;
     carry += tot;
     __syncthreads();
   }
   if (t == 0) totals[slot] = carry;
 }
 
-// Sum of per-chunk zero counts for every sub-level; one block per sub-level.
-__global__ __launch_bounds__(1024) void k_sum_zeros(const uint32_t* ztot, uint64_t nchunks, uint32_t g,
-                                                    uint64_t* zeros_out) {
+template <bool IND>
+__global__ __launch_bounds__(1024) void k_tile_apply(const uint32_t* counts, const uint64_t* tsum,
+                                                     uint64_t* pref, uint64_t nchunks, uint32_t nslot,
+                                                     uint64_t ntiles) {
   __shared__ uint64_t wsum[16];
-  const uint32_t j = blockIdx.x;
+  const uint64_t tile = blockIdx.x;
+  const uint32_t slot = blockIdx.y;
+  const int t = threadIdx.x, lane = t & 63, wid = t >> 6;
+  const uint64_t c = tile * kTile + t;
+  const uint64_t v = c < nchunks ? (IND ? uint64_t(counts[c * nslot + slot] != 0) : counts[c * nslot + slot]) : 0;
+  uint64_t inc = v;
+#pragma unroll
+  for (int d = 1; d < 64; d <<= 1) {
+    const uint64_t x = __shfl_up(inc, d, 64);
+    if (lane >= d) inc += x;
+  }
+  if (lane == 63) wsum[wid] = inc;
+  __syncthreads();
+  uint64_t off = tsum[uint64_t(slot) * ntiles + tile];
+#pragma unroll
+  for (int w = 0; w < 16; ++w)
+    if (w < wid) off += wsum[w];
+  if (c < nchunks) pref[c * nslot + slot] = off + inc - v;
+}
+
+// Sum of per-chunk zero counts of every sub-level (zeros_out pre-cleared);
+// grid (tiles of 1024 chunks, sub-levels).
+__global__ __launch_bounds__(256) void k_sum_zeros(const uint32_t* ztot, uint64_t nchunks, uint32_t g,
+                                                   unsigned long long* zeros_out) {
+  __shared__ uint64_t wsum[4];
+  const uint32_t j = blockIdx.y;
   uint64_t s = 0;
-  for (uint64_t c = threadIdx.x; c < nchunks; c += 1024) s += ztot[c * g + j];
+  for (uint32_t k = threadIdx.x; k < kTile; k += 256) {
+    const uint64_t c = uint64_t(blockIdx.x) * kTile + k;
+    if (c < nchunks) s += ztot[c * g + j];
+  }
 #pragma unroll
   for (int d = 32; d >= 1; d >>= 1) s += __shfl_xor(s, d, 64);
   if ((threadIdx.x & 63) == 0) wsum[threadIdx.x >> 6] = s;
   __syncthreads();
-  if (threadIdx.x == 0) {
-    uint64_t tot = 0;
-    for (int w = 0; w < 16; ++w) tot += wsum[w];
-    zeros_out[j] = tot;
-  }
+  if (threadIdx.x == 0) atomicAdd(&zeros_out[j], (unsigned long long)(wsum[0] + wsum[1] + wsum[2] + wsum[3]));
 }
 
 __host__ __device__ inline uint32_t rev_bits(uint32_t x, uint32_t bits) {
@@ -375,7 +549,7 @@ __host__ __device__ inline uint32_t rev_bits(uint32_t x, uint32_t bits) {
 // has reverse_j(d) == b (matrix order).  Matrix: buckets in index order;
 // tree: buckets in numeric digit order.
 __global__ __launch_bounds__(256) void k_bucket_starts(const uint64_t* totals, uint64_t* gstart,
-                                                       int tree) {
+                                                       int tree, uint64_t* level_total) {
   __shared__ uint64_t tmp[256];
   const uint32_t j = blockIdx.x + 1;
   const uint32_t nb = 1u << j, slot0 = nb - 2;
@@ -392,6 +566,7 @@ __global__ __launch_bounds__(256) void k_bucket_starts(const uint64_t* totals, u
       tmp[i] = run;
       run += v;
     }
+    if (level_total) level_total[j] = run;
   }
   __syncthreads();
   if (t < nb) {
@@ -409,54 +584,7 @@ struct RunGeom {
   uint64_t n;
 };
 
-struct StitchParams {
-  const uint32_t* counts;
-  const uint64_t* pref;
-  const uint64_t* gstart;
-  uint64_t* dst_level[kMaxGroup];    // sub-level j -> destination level (j >= 1)
-  const uint64_t* src_temp;          // chunk-local sub-level j at src_temp + (j-1)*stride
-  uint64_t temp_stride;
-  uint64_t* dst_payload;             // payload plane k at dst_payload + k*dst_pay_stride
-  uint64_t dst_pay_stride;
-  const uint64_t* src_payload;       // chunk-local payload k at src_payload + k*temp_stride
-  uint32_t npay, g, jmax, nslot;
-  uint64_t n, nchunks;
-};
-
-__device__ __forceinline__ uint32_t slot_level(uint32_t slot) {
-  // slot = 2^j - 2 + b  ->  j
-  return 31 - __clz(slot + 2);
-}
-
-// Zero the destination words that a run shares with other runs.
-__global__ void k_zero_edges(StitchParams p) {
-  const uint64_t i = uint64_t(blockIdx.x) * blockDim.x + threadIdx.x;
-  if (i >= p.nchunks * p.nslot) return;
-  const uint64_t c = i / p.nslot;
-  const uint32_t slot = uint32_t(i % p.nslot);
-  const uint64_t len = p.counts[i];
-  if (!len) return;
-  const uint32_t j = slot_level(slot);
-  const uint64_t dst = p.gstart[slot] + p.pref[i];
-  const uint64_t end = dst + len;
-  const bool head = (dst & 63) != 0;
-  const bool tail = (end & 63) != 0 && end != p.n;
-  if (!head && !tail) return;
-  (void)c;
-  if (j < p.g) {
-    uint64_t* d = p.dst_level[j];
-    if (head) d[dst >> 6] = 0;
-    if (tail) d[(end - 1) >> 6] = 0;
-  } else {
-    for (uint32_t k = 0; k < p.npay; ++k) {
-      uint64_t* d = p.dst_payload + k * p.dst_pay_stride;
-      if (head) d[dst >> 6] = 0;
-      if (tail) d[(end - 1) >> 6] = 0;
-    }
-  }
-}
-
-// Read 64 MSB-first bits starting at (possibly negative) bit position s.
+// Read 64 MSB-first bits starting at (possibly negative, > -64) bit position s.
 __device__ __forceinline__ uint64_t window64(const uint64_t* a, int64_t s) {
   const int64_t w = s >> 6;  // arithmetic shift
   const uint32_t sh = uint32_t(s & 63);
@@ -473,92 +601,286 @@ __device__ __forceinline__ uint64_t range_mask(uint32_t a, uint32_t b) {
   return hi & lo;
 }
 
-// Copy one destination word of a run.
-__device__ __forceinline__ void stitch_word(uint64_t* dst, const uint64_t* src, uint64_t dbit,
-                                            uint64_t sbit, uint64_t len, uint64_t w, uint64_t n) {
-  const uint64_t ws = w << 6;
-  const uint64_t a = max(ws, dbit), b = min(ws + 64, dbit + len);
-  const uint64_t v = window64(src, int64_t(sbit) - int64_t(dbit - ws)) &
-                     range_mask(uint32_t(a - ws), uint32_t(b - ws));
-  const uint64_t wend = min(ws + 64, n);
-  if (a == ws && b == wend) dst[w] = v;
-  else if (v) atomicOr(reinterpret_cast<unsigned long long*>(&dst[w]), (unsigned long long)v);
+__device__ __forceinline__ uint32_t slot_level(uint32_t slot) {
+  // slot = 2^j - 2 + b  ->  j
+  return 31 - __clz(slot + 2);
 }
 
-// One block per chunk: runs of the chunk flattened over destination words.
-__global__ __launch_bounds__(256) void k_stitch(StitchParams p) {
-  constexpr int MAXS = (2 << kMaxGroup) - 2;
-  __shared__ uint64_t s_src[MAXS], s_dst[MAXS], s_len[MAXS];
-  __shared__ uint64_t s_wpre[MAXS + 1];
-  __shared__ uint32_t s_mult[MAXS];
-  const uint64_t c = blockIdx.x;
-  const int t = threadIdx.x;
-  const uint32_t ns = p.nslot;
-  // local starts: exclusive scan of counts within each sub-level (tiny)
-  for (uint32_t slot = t; slot < ns; slot += blockDim.x) {
-    const uint32_t j = slot_level(slot);
-    const uint32_t s0 = (1u << j) - 2;
-    uint64_t loc = 0;
-    for (uint32_t b = s0; b < slot; ++b) loc += p.counts[c * ns + b];
-    const uint64_t len = p.counts[c * ns + slot];
-    const uint64_t dst = p.gstart[slot] + p.pref[c * ns + slot];
-    s_src[slot] = c * kChunkSyms + loc;
-    s_dst[slot] = dst;
-    s_len[slot] = len;
-    s_mult[slot] = j < p.g ? 1u : p.npay;
+// Compact run table of one construction pass: for every sub-level j the
+// non-empty (chunk, bucket) runs in destination order, entries at
+// [(2^j - 2) * nchunks, ...).  A run's length is the distance to the next
+// entry's destination (runs of one level tile the level).
+struct RunTable {
+  const uint32_t* counts;
+  const uint64_t* pref;     // exclusive prefix over chunks of counts
+  const uint64_t* gstart;   // global bucket starts
+  const uint64_t* nepref;   // exclusive prefix over chunks of (count != 0)
+  const uint64_t* nebase;   // start of each bucket's entries in its level table
+  uint64_t* tab_dst;
+  uint64_t* tab_src;
+  uint32_t nslot;
+  uint64_t nchunks;
+};
+
+__global__ void k_make_runs(RunTable p) {
+  const uint64_t i = uint64_t(blockIdx.x) * blockDim.x + threadIdx.x;
+  if (i >= p.nchunks * p.nslot) return;
+  const uint64_t c = i / p.nslot;
+  const uint32_t slot = uint32_t(i - c * p.nslot);
+  const uint32_t len = p.counts[i];
+  if (!len) return;
+  const uint32_t j = slot_level(slot);
+  const uint32_t s0 = (1u << j) - 2;
+  uint64_t loc = 0;  // chunk-local start: counts of the chunk's earlier buckets of this level
+  for (uint32_t b = s0; b < slot; ++b) loc += p.counts[c * p.nslot + b];
+  const uint64_t e = uint64_t(s0) * p.nchunks + p.nebase[slot] + p.nepref[i];
+  p.tab_dst[e] = p.gstart[slot] + p.pref[i];
+  p.tab_src[e] = c * kChunkSyms + loc;
+}
+
+// Destination-major gather: each wave writes 64 consecutive words of one
+// level (or of every payload plane), each word exactly once.  The wave finds
+// the run holding its first bit with a 64-way ballot search over the
+// compact table, then walks the following runs.
+struct GatherParams {
+  const uint64_t* tab_dst;  // this sub-level's entries
+  const uint64_t* tab_src;
+  const uint64_t* level_ne; // number of entries (device, index j)
+  uint32_t j;
+  const uint64_t* src;      // chunk-local plane 0 (planes at src + k*src_stride)
+  uint64_t src_stride;
+  uint64_t* dst;            // destination plane 0 (planes at dst + k*dst_stride)
+  uint64_t dst_stride;
+  uint32_t nplanes;
+  uint64_t n;
+};
+
+constexpr uint32_t kGatherWindows = 16;  // 64-word windows per wave
+
+__device__ __forceinline__ uint64_t readlane64(uint64_t v, uint32_t lane) {
+  const uint32_t lo = __builtin_amdgcn_readlane(uint32_t(v), lane);
+  const uint32_t hi = __builtin_amdgcn_readlane(uint32_t(v >> 32), lane);
+  return (uint64_t(hi) << 32) | lo;
+}
+
+// One destination word assembled from the runs cached in the wave (entry i
+// of the cache in lane i: bd = destination start, bs = source start).  All
+// lanes call it (shuffles read 0 from inactive lanes); `live` lanes own a
+// word of the level.
+__device__ __forceinline__ uint64_t gather_word(const uint64_t* src, uint64_t bd, uint64_t bs, uint64_t ws,
+                                                uint64_t we, bool live) {
+  // last cached entry with start <= ws
+  uint32_t idx = 0;
+#pragma unroll
+  for (uint32_t step = 32; step >= 1; step >>= 1) {
+    const uint64_t cand = __shfl(bd, int((idx + step) & 63), 64);
+    if (idx + step < 64 && cand <= ws) idx += step;
+  }
+  uint64_t d0 = __shfl(bd, int(idx), 64);
+  uint64_t d1 = __shfl(bd, int(idx + 1 < 64 ? idx + 1 : 63), 64);
+  uint64_t s0 = __shfl(bs, int(idx), 64);
+  uint64_t v = 0;
+  bool more = live;
+  while (__ballot(more)) {  // wave-uniform loop: shuffles below see every lane
+    if (more) {
+      const uint64_t a = max(d0, ws), b = min(d1, we);
+      if (a < b) {
+        const int64_t sb = int64_t(s0) - int64_t(d0) + int64_t(ws);
+        v |= window64(src, sb) & range_mask(uint32_t(a - ws), uint32_t(b - ws));
+      }
+      if (d1 >= we || idx + 2 > 64) more = false;
+    }
+    const uint32_t nidx = more ? idx + 1 : idx;
+    const uint64_t nd1 = __shfl(bd, int(nidx + 1 < 64 ? nidx + 1 : 63), 64);
+    const uint64_t ns0 = __shfl(bs, int(nidx), 64);
+    if (more) {
+      idx = nidx;
+      d0 = d1;
+      d1 = nd1;
+      s0 = ns0;
+    }
+  }
+  return v;
+}
+
+__global__ __launch_bounds__(256) void k_gather(GatherParams p) {
+  const uint32_t lane = threadIdx.x & 63;
+  const uint64_t W0 = (uint64_t(blockIdx.x) * 4 + (threadIdx.x >> 6)) * 64 * kGatherWindows;
+  const uint64_t X = W0 << 6;
+  if (X >= p.n) return;
+  const uint64_t NE = p.level_ne[p.j];
+  if (NE == 0) return;
+  const uint64_t Xw = min<uint64_t>(X + 4096ull * kGatherWindows, p.n);  // end of this wave's range
+  // last entry with dst <= X (64-way ballot search)
+  uint64_t lo = 0, hi = NE;
+  while (hi - lo > 1) {
+    const uint64_t step = (hi - lo + 63) >> 6;
+    const uint64_t cand = lo + uint64_t(lane + 1) * step;
+    const bool pred = cand < hi && p.tab_dst[cand] <= X;
+    const uint32_t cnt = __popcll(__ballot(pred));
+    lo += uint64_t(cnt) * step;
+    hi = min(hi, lo + step);
+  }
+  const uint64_t bd0 = lo + lane < NE ? p.tab_dst[lo + lane] : p.n;
+  const uint64_t bs0 = lo + lane < NE ? p.tab_src[lo + lane] : 0;
+  // fast path: the 64 cached entries reach past the end of the wave's range
+  const bool cached = __builtin_amdgcn_readlane(uint32_t(bd0 >= Xw), 63) != 0;
+  for (uint32_t k = 0; k < p.nplanes; ++k) {
+    const uint64_t* src = p.src + k * p.src_stride;
+    uint64_t* dst = p.dst + k * p.dst_stride;
+    if (cached) {
+#pragma unroll 4
+      for (uint32_t win = 0; win < kGatherWindows; ++win) {
+        const uint64_t w = W0 + 64ull * win + lane;
+        const uint64_t ws = w << 6;
+        const bool live = ws < p.n;
+        const uint64_t v = gather_word(src, bd0, bs0, ws, min<uint64_t>(ws + 64, p.n), live);
+        if (live) dst[w] = v;
+      }
+      continue;
+    }
+    // general path: walk the entries with a sliding 64-entry cache
+    uint64_t base = lo, e = lo, bd = bd0, bs = bs0;
+    for (uint32_t win = 0; win < kGatherWindows; ++win) {
+      const uint64_t w = W0 + 64ull * win + lane;
+      const uint64_t sw0 = (W0 + 64ull * win) << 6;
+      if (sw0 >= p.n) break;
+      const uint64_t sw1 = min<uint64_t>(sw0 + 4096, p.n);
+      const uint64_t ws = w << 6, we = min<uint64_t>(ws + 64, p.n);
+      uint64_t v = 0;
+      for (;;) {
+        if (e >= NE) break;
+        if (e + 1 - base >= 64) {  // keep entries e and e+1 in the cache
+          base = e;
+          bd = base + lane < NE ? p.tab_dst[base + lane] : p.n;
+          bs = base + lane < NE ? p.tab_src[base + lane] : 0;
+        }
+        const uint64_t d0 = readlane64(bd, uint32_t(e - base));
+        const uint64_t d1 = readlane64(bd, uint32_t(e + 1 - base));  // n past the last entry
+        if (d0 >= sw1) break;
+        if (d1 > sw0) {
+          const uint64_t a = max(d0, ws), b = min(d1, we);
+          if (a < b) {
+            const int64_t sb = int64_t(readlane64(bs, uint32_t(e - base))) - int64_t(d0) + int64_t(ws);
+            v |= window64(src, sb) & range_mask(uint32_t(a - ws), uint32_t(b - ws));
+          }
+        }
+        if (d1 > sw1) break;  // the entry continues into the next window
+        ++e;
+      }
+      if (ws < p.n) dst[w] = v;
+    }
+  }
+}
+
+// Fused offsets for the run table: per (tile of 1024 chunks, slot) the sums of
+// counts and of non-empty indicators ...
+__global__ __launch_bounds__(256) void k_tile_sums2(const uint32_t* counts, uint64_t* tsum, uint64_t* tnz,
+                                                    uint64_t nchunks, uint32_t nslot, uint64_t ntiles) {
+  __shared__ uint64_t ws[2][4];
+  const uint64_t tile = blockIdx.x;
+  const uint32_t slot = blockIdx.y;
+  uint64_t v = 0, z = 0;
+  for (uint32_t k = threadIdx.x; k < kTile; k += 256) {
+    const uint64_t c = tile * kTile + k;
+    if (c < nchunks) {
+      const uint32_t x = counts[c * nslot + slot];
+      v += x;
+      z += x != 0;
+    }
+  }
+#pragma unroll
+  for (int d = 32; d >= 1; d >>= 1) {
+    v += __shfl_xor(v, d, 64);
+    z += __shfl_xor(z, d, 64);
+  }
+  if ((threadIdx.x & 63) == 0) {
+    ws[0][threadIdx.x >> 6] = v;
+    ws[1][threadIdx.x >> 6] = z;
   }
   __syncthreads();
-  if (t == 0) {
-    uint64_t run = 0;
-    for (uint32_t s = 0; s < ns; ++s) {
-      s_wpre[s] = run;
-      const uint64_t len = s_len[s];
-      const uint64_t nwd = len ? ((s_dst[s] + len - 1) >> 6) - (s_dst[s] >> 6) + 1 : 0;
-      run += nwd * s_mult[s];
+  if (threadIdx.x == 0) {
+    tsum[uint64_t(slot) * ntiles + tile] = ws[0][0] + ws[0][1] + ws[0][2] + ws[0][3];
+    tnz[uint64_t(slot) * ntiles + tile] = ws[1][0] + ws[1][1] + ws[1][2] + ws[1][3];
+  }
+}
+
+// ... then (after the tile sums are scanned and the bucket starts known) every
+// non-empty (chunk, bucket) run is written into its level's table.
+__global__ __launch_bounds__(1024) void k_apply_runs(const uint32_t* counts, const uint64_t* tsum,
+                                                     const uint64_t* tnz, const uint64_t* gstart,
+                                                     const uint64_t* nebase, uint64_t* tab_dst, uint64_t* tab_src,
+                                                     uint64_t nchunks, uint32_t nslot, uint64_t ntiles) {
+  __shared__ uint64_t wsum[2][16];
+  const uint64_t tile = blockIdx.x;
+  const uint32_t slot = blockIdx.y;
+  const int t = threadIdx.x, lane = t & 63, wid = t >> 6;
+  const uint64_t c = tile * kTile + t;
+  const uint32_t x = c < nchunks ? counts[c * nslot + slot] : 0;
+  uint64_t iv = x, iz = x != 0;
+#pragma unroll
+  for (int d = 1; d < 64; d <<= 1) {
+    const uint64_t a = __shfl_up(iv, d, 64), b = __shfl_up(iz, d, 64);
+    if (lane >= d) {
+      iv += a;
+      iz += b;
     }
-    s_wpre[ns] = run;
+  }
+  if (lane == 63) {
+    wsum[0][wid] = iv;
+    wsum[1][wid] = iz;
   }
   __syncthreads();
-  const uint64_t total = s_wpre[ns];
-  for (uint64_t i = t; i < total; i += blockDim.x) {
-    // find slot: largest s with s_wpre[s] <= i
-    uint32_t lo = 0, hi = ns;  // invariant s_wpre[lo] <= i < s_wpre[hi]
-    while (hi - lo > 1) {
-      const uint32_t mid = (lo + hi) >> 1;
-      if (s_wpre[mid] <= i) lo = mid;
-      else hi = mid;
+  if (!x) return;
+  uint64_t ov = tsum[uint64_t(slot) * ntiles + tile], oz = tnz[uint64_t(slot) * ntiles + tile];
+#pragma unroll
+  for (int w = 0; w < 16; ++w)
+    if (w < wid) {
+      ov += wsum[0][w];
+      oz += wsum[1][w];
     }
-    const uint32_t s = lo;
-    const uint64_t len = s_len[s], dbit = s_dst[s], sbit = s_src[s];
-    const uint64_t nwd = ((dbit + len - 1) >> 6) - (dbit >> 6) + 1;
-    const uint64_t r = i - s_wpre[s];
-    const uint64_t k = r / nwd, wi = r % nwd;
-    const uint64_t w = (dbit >> 6) + wi;
-    const uint32_t j = slot_level(s);
-    if (j < p.g) {
-      stitch_word(p.dst_level[j], p.src_temp + (j - 1) * p.temp_stride, dbit, sbit, len, w, p.n);
-    } else {
-      stitch_word(p.dst_payload + k * p.dst_pay_stride, p.src_payload + k * p.temp_stride, dbit, sbit,
-                  len, w, p.n);
-    }
-  }
+  const uint64_t pref = ov + iv - x, nzpref = oz + iz - 1;
+  const uint32_t j = slot_level(slot);
+  const uint32_t s0 = (1u << j) - 2;
+  uint64_t loc = 0;  // chunk-local start: counts of the chunk's earlier buckets of this level
+  for (uint32_t b = s0; b < slot; ++b) loc += counts[c * nslot + b];
+  const uint64_t e = uint64_t(s0) * nchunks + nebase[slot] + nzpref;
+  tab_dst[e] = gstart[slot] + pref;
+  tab_src[e] = c * kChunkSyms + loc;
 }
 
 }  // namespace
 
 void launch_scan_slots(hipStream_t s, const uint32_t* counts, uint64_t* pref, uint64_t* totals,
-                       uint64_t nchunks, uint32_t nslot) {
-  hipLaunchKernelGGL(k_scan_slots, dim3(nslot), dim3(1024), 0, s, counts, pref, totals, nchunks, nslot);
+                       uint64_t nchunks, uint32_t nslot, uint64_t* tile_scratch, bool indicator) {
+  const uint64_t ntiles = (nchunks + kTile - 1) / kTile;
+  if (indicator)
+    hipLaunchKernelGGL(k_tile_sums<true>, dim3(uint32_t(ntiles), nslot), dim3(256), 0, s, counts, tile_scratch,
+                       nchunks, nslot, ntiles);
+  else
+    hipLaunchKernelGGL(k_tile_sums<false>, dim3(uint32_t(ntiles), nslot), dim3(256), 0, s, counts, tile_scratch,
+                       nchunks, nslot, ntiles);
+  hipLaunchKernelGGL(k_tile_scan, dim3(nslot), dim3(1024), 0, s, tile_scratch, totals, ntiles);
+  if (indicator)
+    hipLaunchKernelGGL(k_tile_apply<true>, dim3(uint32_t(ntiles), nslot), dim3(1024), 0, s, counts, tile_scratch,
+                       pref, nchunks, nslot, ntiles);
+  else
+    hipLaunchKernelGGL(k_tile_apply<false>, dim3(uint32_t(ntiles), nslot), dim3(1024), 0, s, counts, tile_scratch,
+                       pref, nchunks, nslot, ntiles);
   PWM_HIP_TRY(hipGetLastError());
+}
+
+uint64_t scan_scratch_words(uint64_t nchunks, uint32_t nslot) {
+  return ((nchunks + kTile - 1) / kTile) * std::max<uint32_t>(nslot, 1);
 }
 
 namespace {
 
 template <int NP>
 void launch_k1(pwmhip_ctx* ctx, const K1Params& prm) {
-  const int per_cu = (NP <= 4) ? 2 : 1;
-  const uint64_t grid = std::min<uint64_t>(prm.nchunks, uint64_t(ctx->num_cus) * per_cu);
-  hipLaunchKernelGGL(k_local_build<NP>, dim3(uint32_t(grid)), dim3(P), 0, ctx->stream, prm);
+  constexpr int WAVES = waves_for<NP>();
+  const uint64_t grid = std::min<uint64_t>((prm.nchunks + WAVES - 1) / WAVES, uint64_t(ctx->num_cus));
+  hipLaunchKernelGGL(k_local_build<NP>, dim3(uint32_t(grid)), dim3(64 * WAVES), 0, ctx->stream, prm);
   PWM_HIP_TRY(hipGetLastError());
 }
 
@@ -624,9 +946,13 @@ void build_plain_device(pwmhip_ctx* ctx, const void* d_text, uint64_t n, uint32_
   const size_t sz_pay = std::max<size_t>(1, size_t(npay0) * stride);
   const size_t sz_pong = std::max<size_t>(1, size_t(npay0 > kMaxGroup ? npay0 - kMaxGroup : 0) * stride);
   const size_t sz_cnt = std::max<size_t>(1, nchunks * nslot_max);
+  const size_t sz_tile = scan_scratch_words(nchunks, nslot_max);
+  const size_t sz_tab = std::max<size_t>(1, nchunks * nslot_max);
   auto carve = [&](arena& A, uint64_t*& temp_levels, uint64_t*& temp_payload, uint64_t*& S_ping,
                    uint64_t*& S_pong, uint32_t*& counts, uint64_t*& pref, uint32_t*& ztot, uint64_t*& totals,
-                   uint64_t*& gstart, uint64_t*& zeros_dev, unsigned long long*& sym_or) {
+                   uint64_t*& gstart, uint64_t*& zeros_dev, unsigned long long*& sym_or, uint64_t*& tiles,
+                   uint64_t*& nepref, uint64_t*& netot, uint64_t*& nebase, uint64_t*& level_ne, uint64_t*& tab_dst,
+                   uint64_t*& tab_src) {
     temp_levels = A.take<uint64_t>(sz_temp);
     temp_payload = A.take<uint64_t>(sz_pay);
     S_ping = A.take<uint64_t>(sz_pay);
@@ -638,21 +964,30 @@ void build_plain_device(pwmhip_ctx* ctx, const void* d_text, uint64_t n, uint32_
     gstart = A.take<uint64_t>(nslot_max);
     zeros_dev = A.take<uint64_t>(64);
     sym_or = A.take<unsigned long long>(1);
+    tiles = A.take<uint64_t>(sz_tile);
+    nepref = A.take<uint64_t>(sz_cnt);
+    netot = A.take<uint64_t>(nslot_max);
+    nebase = A.take<uint64_t>(nslot_max);
+    level_ne = A.take<uint64_t>(kMaxGroup + 1);
+    tab_dst = A.take<uint64_t>(sz_tab);
+    tab_src = A.take<uint64_t>(sz_tab);
   };
-  uint64_t *temp_levels, *temp_payload, *S_ping, *S_pong, *pref, *totals, *gstart, *zeros_dev;
+  uint64_t *temp_levels, *temp_payload, *S_ping, *S_pong, *pref, *totals, *gstart, *zeros_dev, *tiles;
+  uint64_t *nepref, *netot, *nebase, *level_ne, *tab_dst, *tab_src;
   uint32_t *counts, *ztot;
   unsigned long long* sym_or;
   {
     arena sizing;  // dry run on an unbounded arena to measure
     sizing.cap = ~size_t(0) >> 1;
     carve(sizing, temp_levels, temp_payload, S_ping, S_pong, counts, pref, ztot, totals, gstart, zeros_dev,
-          sym_or);
+          sym_or, tiles, nepref, netot, nebase, level_ne, tab_dst, tab_src);
     ctx->scratch.reset();
     ctx->scratch.ensure(sizing.used);
   }
   carve(ctx->scratch, temp_levels, temp_payload, S_ping, S_pong, counts, pref, ztot, totals, gstart, zeros_dev,
-        sym_or);
+        sym_or, tiles, nepref, netot, nebase, level_ne, tab_dst, tab_src);
   PWM_HIP_TRY(hipMemsetAsync(sym_or, 0, 8, s));
+  PWM_HIP_TRY(hipMemsetAsync(zeros_dev, 0, 64 * 8, s));
 
   PWM_HIP_TRY(hipEventRecord(ctx->ev[0], s));
   const uint64_t* S_in = nullptr;
@@ -682,37 +1017,51 @@ void build_plain_device(pwmhip_ctx* ctx, const void* d_text, uint64_t n, uint32_
     k1.temp_stride = stride;
     k1.counts = counts;
     k1.nslot = nslot;
-    k1.ztot = ztot;
+    k1.zeros_out = reinterpret_cast<unsigned long long*>(zeros_dev + l0);
     k1.sym_or = sym_or;
     k1.nchunks = nchunks;
+    k1.aligned16 = (reinterpret_cast<uintptr_t>(d_text) & 15) == 0;
     dispatch_k1(ctx, k1);
     if (gi == 0) PWM_HIP_TRY(hipEventRecord(ctx->ev[1], s));
 
-    hipLaunchKernelGGL(k_sum_zeros, dim3(g), dim3(1024), 0, s, ztot, nchunks, g, zeros_dev + l0);
     if (nslot) {
-      hipLaunchKernelGGL(k_scan_slots, dim3(nslot), dim3(1024), 0, s, counts, pref, totals, nchunks, nslot);
-      hipLaunchKernelGGL(k_bucket_starts, dim3(jmax), dim3(256), 0, s, totals, gstart,
-                         tree_direct ? 1 : 0);
-      StitchParams sp{};
-      sp.counts = counts;
-      sp.pref = pref;
-      sp.gstart = gstart;
-      for (uint32_t j = 1; j < g; ++j) sp.dst_level[j] = ws->d_words + ws->level_offset[l0 + j];
-      sp.src_temp = temp_levels;
-      sp.temp_stride = stride;
-      sp.dst_payload = S_out;
-      sp.dst_pay_stride = stride;
-      sp.src_payload = temp_payload;
-      sp.npay = npay;
-      sp.g = g;
-      sp.jmax = jmax;
-      sp.nslot = nslot;
-      sp.n = n;
-      sp.nchunks = nchunks;
-      const uint64_t nthreads = nchunks * nslot;
-      hipLaunchKernelGGL(k_zero_edges, dim3(uint32_t((nthreads + 255) / 256)), dim3(256), 0, s, sp);
+      const uint64_t ntiles = (nchunks + kTile - 1) / kTile;
+      uint64_t* tsum = tiles;
+      uint64_t* tnz = pref;  // tile sums of the indicators (pref is free scratch here)
+      hipLaunchKernelGGL(k_tile_sums2, dim3(uint32_t(ntiles), nslot), dim3(256), 0, s, counts, tsum, tnz, nchunks,
+                         nslot, ntiles);
+      hipLaunchKernelGGL(k_tile_scan, dim3(nslot), dim3(1024), 0, s, tsum, totals, ntiles);
+      hipLaunchKernelGGL(k_tile_scan, dim3(nslot), dim3(1024), 0, s, tnz, netot, ntiles);
+      hipLaunchKernelGGL(k_bucket_starts, dim3(jmax), dim3(256), 0, s, totals, gstart, tree_direct ? 1 : 0,
+                         (uint64_t*)nullptr);
+      hipLaunchKernelGGL(k_bucket_starts, dim3(jmax), dim3(256), 0, s, netot, nebase, tree_direct ? 1 : 0,
+                         level_ne);
+      hipLaunchKernelGGL(k_apply_runs, dim3(uint32_t(ntiles), nslot), dim3(1024), 0, s, counts, tsum, tnz, gstart,
+                         nebase, tab_dst, tab_src, nchunks, nslot, ntiles);
       if (gi == 0) PWM_HIP_TRY(hipEventRecord(ctx->ev[2], s));
-      hipLaunchKernelGGL(k_stitch, dim3(uint32_t(nchunks)), dim3(256), 0, s, sp);
+      const uint64_t W = words_for(n);
+      const uint32_t gblocks = uint32_t((W + 256 * kGatherWindows - 1) / (256 * kGatherWindows));
+      for (uint32_t j = 1; j <= jmax; ++j) {
+        GatherParams gp{};
+        const uint64_t toff = uint64_t((1u << j) - 2) * nchunks;
+        gp.tab_dst = tab_dst + toff;
+        gp.tab_src = tab_src + toff;
+        gp.level_ne = level_ne;
+        gp.j = j;
+        gp.n = n;
+        if (j < g) {
+          gp.src = temp_levels + (j - 1) * stride;
+          gp.dst = ws->d_words + ws->level_offset[l0 + j];
+          gp.nplanes = 1;
+        } else {
+          gp.src = temp_payload;
+          gp.src_stride = stride;
+          gp.dst = S_out;
+          gp.dst_stride = stride;
+          gp.nplanes = npay;
+        }
+        hipLaunchKernelGGL(k_gather, dim3(gblocks), dim3(256), 0, s, gp);
+      }
     } else if (gi == 0) {
       PWM_HIP_TRY(hipEventRecord(ctx->ev[2], s));
     }

@@ -500,7 +500,7 @@ void build_huffman_device(pwmhip_ctx* ctx, const void* d_text, uint64_t n, uint3
   auto add = [&](size_t b) { need += (b + 255) & ~size_t(255); };
   add(S * 4); add(S * 8); add(S * L * 4); add(nslot * 4 + 4); add((L + 1) * 4); add(nslot * 8 + 8);
   add(L * tstride * 8); add(nchunks * std::max<uint32_t>(nslot, 1) * 4); add(nchunks * std::max<uint32_t>(nslot, 1) * 8);
-  add(nslot * 8 + 8); add(L * 8); add(L * 8);
+  add(nslot * 8 + 8); add(L * 8); add(L * 8); add(scan_scratch_words(nchunks, nslot) * 8);
   ctx->scratch.reset();
   ctx->scratch.ensure(need);
   arena& A = ctx->scratch;
@@ -516,6 +516,7 @@ void build_huffman_device(pwmhip_ctx* ctx, const void* d_text, uint64_t n, uint3
   uint64_t* d_tot = A.take<uint64_t>(nslot + 1);
   uint64_t** d_dst = A.take<uint64_t*>(L);
   uint64_t* d_lbits = A.take<uint64_t>(L);
+  uint64_t* d_tiles = A.take<uint64_t>(scan_scratch_words(nchunks, nslot));
   std::vector<uint32_t> len32(S);
   for (uint64_t s = 0; s < S; ++s) len32[s] = uint32_t(ct.len[s]);
   std::vector<uint64_t*> dst(L);
@@ -555,7 +556,7 @@ void build_huffman_device(pwmhip_ctx* ctx, const void* d_text, uint64_t n, uint3
   PWM_HIP_TRY(hipGetLastError());
   PWM_HIP_TRY(hipEventRecord(ctx->ev[1], st));
   if (nslot) {
-    launch_scan_slots(st, d_counts, d_pref, d_tot, nchunks, nslot);
+    launch_scan_slots(st, d_counts, d_pref, d_tot, nchunks, nslot, d_tiles);
     PWM_HIP_TRY(hipEventRecord(ctx->ev[2], st));
     HStitch hs{};
     hs.counts = d_counts;

@@ -35,9 +35,9 @@ struct status_error {
 };
 
 // ---- geometry ---------------------------------------------------------------
-// A chunk is the unit of the per-workgroup local build: CHUNK_WORDS 64-bit
-// words per bit plane, i.e. CHUNK_WORDS*64 symbols.
-constexpr int kChunkWords = 256;
+// A chunk is the unit of the local build (one wave): kChunkWords 64-bit
+// words per bit plane (one per lane), i.e. kChunkWords*64 symbols.
+constexpr int kChunkWords = 64;
 constexpr uint64_t kChunkSyms = uint64_t(kChunkWords) * 64;
 constexpr int kMaxGroup = 8;  // levels built per pass (2^8 buckets per chunk)
 
@@ -98,8 +98,11 @@ void alloc_structure(pwmhip_ws* ws, uint64_t levels, const uint64_t* bits,
                      uint64_t* reuse = nullptr, uint64_t reuse_bytes = 0);
 
 // Exclusive scan over chunks of counts[c*nslot + slot] (one block per slot).
+// tile_scratch holds scan_scratch_words(nchunks, nslot) words.
+// indicator: scan (count != 0) instead of count.
 void launch_scan_slots(hipStream_t s, const uint32_t* counts, uint64_t* pref, uint64_t* totals,
-                       uint64_t nchunks, uint32_t nslot);
+                       uint64_t nchunks, uint32_t nslot, uint64_t* tile_scratch, bool indicator = false);
+uint64_t scan_scratch_words(uint64_t nchunks, uint32_t nslot);
 
 // Huffman-shaped construction (huffman.hip).
 void build_huffman_device(pwmhip_ctx* ctx, const void* d_text, uint64_t n, uint32_t width,
