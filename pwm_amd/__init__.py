@@ -280,6 +280,70 @@ def build(text, levels, algo="ppc", is_tree=False, ctx=None, check_range=False, 
     return WaveletStructure(ctx, ws)
 
 
+class _BatchBlock:
+    """Device memory shared by the structures of one build_batch call; freed
+    when the last of them is gone."""
+
+    def __init__(self, ctx, ws):
+        self._ctx = ctx
+        self._ws = ws
+
+    def __del__(self):
+        try:
+            if self._ws is not None and self._ctx.h:
+                _lib.pwmhip_ws_free(self._ctx.h, ctypes.byref(self._ws))
+        except Exception:
+            pass
+        self._ws = None
+
+
+def build_batch(texts, levels, algo="ppc", is_tree=False, ctx=None, check_range=False):
+    """pwmhip_build_batch: construct one structure per text in one launch per
+    phase.  `texts` is a list of numpy arrays (host) or of device tensors
+    (all of one kind and symbol width).  Returns a list of WaveletStructure;
+    each is equal to build(text, ...) of its text."""
+    ctx = ctx or default_context()
+    T = len(texts)
+    if T == 0:
+        return []
+    args = [_text_arg(t) for t in texts]
+    ptrs = (ctypes.c_void_p * T)(*[a[0].value for a in args])
+    sizes = (ctypes.c_uint64 * T)(*[a[1] for a in args])
+    widths = {a[2] for a in args}
+    kinds = {a[3] for a in args}
+    if len(widths) != 1 or len(kinds) != 1:
+        raise PwmError(ERR_INVALID, "build_batch: texts must share symbol width and location")
+    keep = [a[4] if len(a) > 4 else t for a, t in zip(args, texts)]
+    flags = kinds.pop()
+    if is_tree:
+        flags |= FLAG_TREE
+    if check_range:
+        flags |= FLAG_CHECK_RANGE
+    outs = (_WS * T)()
+    st = _lib.pwmhip_build_batch(ctx.h, ptrs, sizes, T, widths.pop(), levels, ALGOS[algo], flags, outs)
+    del keep
+    _check(st, "build_batch %s" % algo)
+    res = []
+    block = None
+    if outs[0].alloc_bytes and (T == 1 or outs[1].alloc_bytes == 0):
+        # one shared allocation, owned by the first structure: hand it to a
+        # block object and make every structure a view that keeps it alive
+        owner = _WS()
+        ctypes.memmove(ctypes.byref(owner), ctypes.byref(outs[0]), ctypes.sizeof(_WS))
+        owner.code_len = None
+        owner.code_word = None
+        owner.n_codes = 0
+        outs[0].alloc_bytes = 0
+        block = _BatchBlock(ctx, owner)
+    for t in range(T):
+        w = _WS()
+        ctypes.memmove(ctypes.byref(w), ctypes.byref(outs[t]), ctypes.sizeof(_WS))
+        ws = WaveletStructure(ctx, w)
+        ws._block = block
+        res.append(ws)
+    return res
+
+
 class ConstructionAlgorithm:
     """Mirror of construction_algorithm (algorithm.hpp:83-191) for one GPU
     algorithm; `name` follows the reference's registrations

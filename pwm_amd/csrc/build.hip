@@ -37,6 +37,7 @@
 
 #include "pwm_internal.hpp"
 
+#include <cstring>
 #include <algorithm>
 
 namespace pwm {
@@ -163,7 +164,51 @@ struct K1Params {
   unsigned long long* sym_or;
   uint64_t nchunks;
   int aligned16;             // byte text 16-byte aligned: vector loads + prefetch
+  // batch mode (b_chunk_base != nullptr): text t owns chunks
+  // [b_chunk_base[t], b_chunk_base[t+1]); its level 0 starts at word
+  // b_out0[t] of out_level0 and its zero counts at b_zeros[t*64]
+  const uint64_t* b_chunk_base;
+  const uint64_t* b_text;
+  const uint64_t* b_n;
+  const uint64_t* b_out0;
+  unsigned long long* b_zeros;
+  uint32_t ntexts;
+  unsigned int* work;        // dynamic chunk queue (atomic counter), or nullptr
 };
+
+// Where a chunk's symbols come from and where its level-l0 word goes.
+struct ChunkView {
+  const void* text;          // the chunk's text
+  uint64_t base;             // first symbol of the chunk inside its text
+  uint64_t n;                // the text's length
+  uint64_t* lvl0;            // level-l0 word of the chunk's first 64 symbols
+  unsigned long long* zout;  // zero counters of the chunk's text
+};
+
+__device__ __forceinline__ ChunkView chunk_view(const K1Params& p, uint64_t c) {
+  ChunkView v;
+  if (!p.b_chunk_base) {
+    v.text = p.text;
+    v.base = c * kChunkSyms;
+    v.n = p.n;
+    v.lvl0 = p.out_level0 + c * kChunkWords;
+    v.zout = p.zeros_out;
+    return v;
+  }
+  uint32_t lo = 0, hi = p.ntexts;  // chunk_base[lo] <= c < chunk_base[hi]
+  while (hi - lo > 1) {
+    const uint32_t mid = (lo + hi) >> 1;
+    if (p.b_chunk_base[mid] <= c) lo = mid;
+    else hi = mid;
+  }
+  const uint64_t lc = c - p.b_chunk_base[lo];
+  v.text = reinterpret_cast<const void*>(p.b_text[lo]);
+  v.base = lc * kChunkSyms;
+  v.n = p.b_n[lo];
+  v.lvl0 = p.out_level0 + p.b_out0[lo] + lc * kChunkWords;
+  v.zout = p.b_zeros + uint64_t(lo) * 64;
+  return v;
+}
 
 // Gather bit s (0 = least significant) of 4 packed bytes into 4 bits.
 // (v_dot4_u32_u8 multiplies the masked bytes by their place values 1,2,4,8.)
@@ -172,10 +217,10 @@ __device__ __forceinline__ uint32_t gather4(uint32_t x4, uint32_t s) {
 }
 
 template <int NP, typename T>
-__device__ __forceinline__ void load_planes_text(const K1Params& p, uint64_t e0, int cnt,
+__device__ __forceinline__ void load_planes_text(const K1Params& p, const void* text, uint64_t e0, int cnt,
                                                  uint64_t* acc, uint64_t& my_or) {
   // symbol bit of plane k: s_k = L-1-l0-k ; byte index s_k>>3, bit s_k&7
-  const T* tx = static_cast<const T*>(p.text);
+  const T* tx = static_cast<const T*>(text);
 #pragma unroll 1
   for (int q = 0; q < 16; ++q) {
     uint64_t v[4];
@@ -309,24 +354,39 @@ __global__ __launch_bounds__(64 * waves_for<NP>()) void k_local_build(K1Params p
 
   const uint64_t gw = uint64_t(blockIdx.x) * WAVES + wid;
   const uint64_t tw = uint64_t(gridDim.x) * WAVES;
-  const bool fast8 = !p.planes_in && p.width == 1 && p.aligned16;
-  auto full_chunk = [&](uint64_t cc) { return (cc + 1) * kChunkSyms <= p.n; };
-  auto load_chunk = [&](uint64_t cc, uint4* d) {
-    const uint4* src = reinterpret_cast<const uint4*>(static_cast<const uint8_t*>(p.text) +
-                                                      cc * kChunkSyms + 64ull * lane);
+  const bool batch = p.b_chunk_base != nullptr;
+  // next chunk of this wave: static round robin, or the device work queue
+  auto next_chunk = [&](uint64_t prev, bool first) -> uint64_t {
+    if (!p.work) return first ? gw : prev + tw;
+    uint32_t c = 0;
+    if (lane == 0) c = atomicAdd(p.work, 1u);
+    return __builtin_amdgcn_readfirstlane(c);
+  };
+  const bool fast_mode = !p.planes_in && p.width == 1 && p.aligned16;
+  auto full_chunk = [&](const ChunkView& v) { return v.base + kChunkSyms <= v.n; };
+  auto load_chunk = [&](const ChunkView& v, uint4* d) {
+    const uint4* src =
+        reinterpret_cast<const uint4*>(static_cast<const uint8_t*>(v.text) + v.base + 64ull * lane);
     d[0] = src[0];
     d[1] = src[1];
     d[2] = src[2];
     d[3] = src[3];
   };
   uint4 pf[4];
-  if (fast8 && gw < p.nchunks && full_chunk(gw)) load_chunk(gw, pf);
+  uint64_t c = next_chunk(0, true);
+  ChunkView cv;
+  if (c < p.nchunks) {
+    cv = chunk_view(p, c);
+    if (fast_mode && full_chunk(cv)) load_chunk(cv, pf);
+  }
 
-  for (uint64_t c = gw; c < p.nchunks; c += tw) {
+  while (c < p.nchunks) {
     wave_sync();
-    const bool have_cur = fast8 && full_chunk(c);
-    const uint64_t base = c * kChunkSyms;
-    const uint32_t Ec = uint32_t(min<uint64_t>(kChunkSyms, p.n - base));
+    const uint64_t cn = next_chunk(c, false);
+    ChunkView nv;
+    if (cn < p.nchunks) nv = chunk_view(p, cn);
+    const bool have_cur = fast_mode && full_chunk(cv);
+    const uint32_t Ec = uint32_t(min<uint64_t>(kChunkSyms, cv.n - cv.base));
     const uint32_t nw = (Ec + 63) >> 6;
     const int cnt = max(0, min(64, int(Ec) - 64 * lane));
 
@@ -341,16 +401,18 @@ __global__ __launch_bounds__(64 * waves_for<NP>()) void k_local_build(K1Params p
       }
     } else if (have_cur) {
       planes_from_bytes<NP>(p, pf, acc, my_or);
-      const uint64_t cn = c + tw;  // prefetch the next chunk of this wave
-      if (cn < p.nchunks && full_chunk(cn)) load_chunk(cn, pf);
+      if (cn < p.nchunks && fast_mode && full_chunk(nv)) load_chunk(nv, pf);  // prefetch
     } else if (cnt > 0) {
-      const uint64_t e0 = base + 64ull * lane;
+      const uint64_t e0 = cv.base + 64ull * lane;
       switch (p.width) {
-        case 1: load_planes_text<NP, uint8_t>(p, e0, cnt, acc, my_or); break;
-        case 2: load_planes_text<NP, uint16_t>(p, e0, cnt, acc, my_or); break;
-        case 4: load_planes_text<NP, uint32_t>(p, e0, cnt, acc, my_or); break;
-        default: load_planes_text<NP, uint64_t>(p, e0, cnt, acc, my_or); break;
+        case 1: load_planes_text<NP, uint8_t>(p, cv.text, e0, cnt, acc, my_or); break;
+        case 2: load_planes_text<NP, uint16_t>(p, cv.text, e0, cnt, acc, my_or); break;
+        case 4: load_planes_text<NP, uint32_t>(p, cv.text, e0, cnt, acc, my_or); break;
+        default: load_planes_text<NP, uint64_t>(p, cv.text, e0, cnt, acc, my_or); break;
       }
+      if (cn < p.nchunks && fast_mode && full_chunk(nv)) load_chunk(nv, pf);
+    } else if (cn < p.nchunks && fast_mode && full_chunk(nv)) {
+      load_chunk(nv, pf);
     }
 #pragma unroll
     for (int k = 0; k < NP; ++k)
@@ -367,7 +429,7 @@ __global__ __launch_bounds__(64 * waves_for<NP>()) void k_local_build(K1Params p
     for (uint32_t j = 0; j < p.g; ++j) {
       const uint64_t m = L.buf[cur][j][lane];
       if (lane < int(nw)) {
-        if (j == 0) p.out_level0[c * P + lane] = brev64(m);
+        if (j == 0) cv.lvl0[lane] = brev64(m);
         else p.temp_levels[(j - 1) * p.temp_stride + c * P + lane] = brev64(m);
       }
       const uint64_t m1 = m & vm, m0 = ~m & vm;
@@ -376,7 +438,11 @@ __global__ __launch_bounds__(64 * waves_for<NP>()) void k_local_build(K1Params p
       const uint32_t zoff = zinc - z, Zt = __builtin_amdgcn_readlane(zinc, 63);
       const uint32_t oof = min(64u * lane, Ec) - zoff;  // every earlier word is full
       const uint32_t Ot = Ec - Zt;
-      zsum[j] += Zt;
+      if (batch) {
+        if (lane == 0 && Zt) atomicAdd(cv.zout + j, (unsigned long long)Zt);
+      } else {
+        zsum[j] += Zt;
+      }
       const bool more = (j + 1 < p.g) || (p.np > p.g);
       if (!more) break;
       L.ooff[lane] = oof;
@@ -428,8 +494,10 @@ __global__ __launch_bounds__(64 * waves_for<NP>()) void k_local_build(K1Params p
         if (k >= int(p.g) && k < int(p.np))
           p.temp_payload[(k - p.g) * p.temp_stride + c * P + lane] = brev64(L.buf[cur][k][lane]);
     }
+    c = cn;
+    cv = nv;
   }
-  if (lane == 0) {
+  if (lane == 0 && !batch) {
 #pragma unroll
     for (int j = 0; j < kMaxGroup; ++j)
       if (j < int(p.g) && zsum[j]) atomicAdd(p.zeros_out + j, (unsigned long long)zsum[j]);
@@ -703,38 +771,40 @@ __device__ __forceinline__ uint64_t gather_word(const uint64_t* src, uint64_t bd
   return v;
 }
 
-__global__ __launch_bounds__(256) void k_gather(GatherParams p) {
+// Writes the wave's destination words [W0, W0 + 64*kGatherWindows) of `nplanes`
+// planes of one level from the level's run table (tab_dst/tab_src, NE
+// entries, destination order).
+__device__ void gather_range(const uint64_t* tab_dst, const uint64_t* tab_src, uint64_t NE, const uint64_t* src0,
+                             uint64_t src_stride, uint64_t* dst0, uint64_t dst_stride, uint32_t nplanes, uint64_t n,
+                             uint64_t W0) {
   const uint32_t lane = threadIdx.x & 63;
-  const uint64_t W0 = (uint64_t(blockIdx.x) * 4 + (threadIdx.x >> 6)) * 64 * kGatherWindows;
   const uint64_t X = W0 << 6;
-  if (X >= p.n) return;
-  const uint64_t NE = p.level_ne[p.j];
-  if (NE == 0) return;
-  const uint64_t Xw = min<uint64_t>(X + 4096ull * kGatherWindows, p.n);  // end of this wave's range
+  if (X >= n || NE == 0) return;
+  const uint64_t Xw = min<uint64_t>(X + 4096ull * kGatherWindows, n);  // end of this wave's range
   // last entry with dst <= X (64-way ballot search)
   uint64_t lo = 0, hi = NE;
   while (hi - lo > 1) {
     const uint64_t step = (hi - lo + 63) >> 6;
     const uint64_t cand = lo + uint64_t(lane + 1) * step;
-    const bool pred = cand < hi && p.tab_dst[cand] <= X;
+    const bool pred = cand < hi && tab_dst[cand] <= X;
     const uint32_t cnt = __popcll(__ballot(pred));
     lo += uint64_t(cnt) * step;
     hi = min(hi, lo + step);
   }
-  const uint64_t bd0 = lo + lane < NE ? p.tab_dst[lo + lane] : p.n;
-  const uint64_t bs0 = lo + lane < NE ? p.tab_src[lo + lane] : 0;
+  const uint64_t bd0 = lo + lane < NE ? tab_dst[lo + lane] : n;
+  const uint64_t bs0 = lo + lane < NE ? tab_src[lo + lane] : 0;
   // fast path: the 64 cached entries reach past the end of the wave's range
   const bool cached = __builtin_amdgcn_readlane(uint32_t(bd0 >= Xw), 63) != 0;
-  for (uint32_t k = 0; k < p.nplanes; ++k) {
-    const uint64_t* src = p.src + k * p.src_stride;
-    uint64_t* dst = p.dst + k * p.dst_stride;
+  for (uint32_t k = 0; k < nplanes; ++k) {
+    const uint64_t* src = src0 + k * src_stride;
+    uint64_t* dst = dst0 + k * dst_stride;
     if (cached) {
 #pragma unroll 4
       for (uint32_t win = 0; win < kGatherWindows; ++win) {
         const uint64_t w = W0 + 64ull * win + lane;
         const uint64_t ws = w << 6;
-        const bool live = ws < p.n;
-        const uint64_t v = gather_word(src, bd0, bs0, ws, min<uint64_t>(ws + 64, p.n), live);
+        const bool live = ws < n;
+        const uint64_t v = gather_word(src, bd0, bs0, ws, min<uint64_t>(ws + 64, n), live);
         if (live) dst[w] = v;
       }
       continue;
@@ -744,16 +814,16 @@ __global__ __launch_bounds__(256) void k_gather(GatherParams p) {
     for (uint32_t win = 0; win < kGatherWindows; ++win) {
       const uint64_t w = W0 + 64ull * win + lane;
       const uint64_t sw0 = (W0 + 64ull * win) << 6;
-      if (sw0 >= p.n) break;
-      const uint64_t sw1 = min<uint64_t>(sw0 + 4096, p.n);
-      const uint64_t ws = w << 6, we = min<uint64_t>(ws + 64, p.n);
+      if (sw0 >= n) break;
+      const uint64_t sw1 = min<uint64_t>(sw0 + 4096, n);
+      const uint64_t ws = w << 6, we = min<uint64_t>(ws + 64, n);
       uint64_t v = 0;
       for (;;) {
         if (e >= NE) break;
         if (e + 1 - base >= 64) {  // keep entries e and e+1 in the cache
           base = e;
-          bd = base + lane < NE ? p.tab_dst[base + lane] : p.n;
-          bs = base + lane < NE ? p.tab_src[base + lane] : 0;
+          bd = base + lane < NE ? tab_dst[base + lane] : n;
+          bs = base + lane < NE ? tab_src[base + lane] : 0;
         }
         const uint64_t d0 = readlane64(bd, uint32_t(e - base));
         const uint64_t d1 = readlane64(bd, uint32_t(e + 1 - base));  // n past the last entry
@@ -768,9 +838,48 @@ __global__ __launch_bounds__(256) void k_gather(GatherParams p) {
         if (d1 > sw1) break;  // the entry continues into the next window
         ++e;
       }
-      if (ws < p.n) dst[w] = v;
+      if (ws < n) dst[w] = v;
     }
   }
+}
+
+__global__ __launch_bounds__(256) void k_gather(GatherParams p) {
+  const uint64_t W0 = (uint64_t(blockIdx.x) * 4 + (threadIdx.x >> 6)) * 64 * kGatherWindows;
+  gather_range(p.tab_dst, p.tab_src, p.level_ne[p.j], p.src, p.src_stride, p.dst, p.dst_stride, p.nplanes, p.n,
+               W0);
+}
+
+// Batch: waves are assigned to texts by wave_base (per text), level j.
+struct BatchGatherParams {
+  const uint64_t* wave_base;   // ntexts + 1
+  const uint64_t* chunk_base;  // ntexts + 1
+  const uint64_t* text_n;
+  const uint64_t* out0;        // word offset of each text's level 0
+  const uint64_t* stride;      // words between a text's levels
+  const uint64_t* level_ne;    // [t * 8 + j]
+  const uint64_t* tab_dst;
+  const uint64_t* tab_src;
+  const uint64_t* src;         // chunk-local level j at src + (j-1)*src_stride
+  uint64_t src_stride;
+  uint64_t* out;
+  uint32_t ntexts, j;
+  uint64_t nchunks;
+};
+
+__global__ __launch_bounds__(256) void k_gather_batch(BatchGatherParams p) {
+  const uint64_t wv = uint64_t(blockIdx.x) * 4 + (threadIdx.x >> 6);
+  if (wv >= p.wave_base[p.ntexts]) return;
+  uint32_t lo = 0, hi = p.ntexts;  // wave_base[lo] <= wv < wave_base[hi]
+  while (hi - lo > 1) {
+    const uint32_t mid = (lo + hi) >> 1;
+    if (p.wave_base[mid] <= wv) lo = mid;
+    else hi = mid;
+  }
+  const uint32_t t = lo;
+  const uint64_t W0 = (wv - p.wave_base[t]) * 64 * kGatherWindows;
+  const uint64_t toff = uint64_t((1u << p.j) - 2) * p.nchunks + p.chunk_base[t] * (1ull << p.j);
+  gather_range(p.tab_dst + toff, p.tab_src + toff, p.level_ne[uint64_t(t) * 8 + p.j],
+               p.src + (p.j - 1) * p.src_stride, 0, p.out + p.out0[t] + p.j * p.stride[t], 0, 1, p.text_n[t], W0);
 }
 
 // Fused offsets for the run table: per (tile of 1024 chunks, slot) the sums of
@@ -847,6 +956,78 @@ __global__ __launch_bounds__(1024) void k_apply_runs(const uint32_t* counts, con
   const uint64_t e = uint64_t(s0) * nchunks + nebase[slot] + nzpref;
   tab_dst[e] = gstart[slot] + pref;
   tab_src[e] = c * kChunkSyms + loc;
+}
+
+// ---- batch offsets --------------------------------------------------------
+// Per text and slot: totals of counts and of non-empty indicators, as
+// differences of the global exclusive prefixes at the text's chunk range.
+__global__ void k_text_totals(const uint64_t* pref, const uint64_t* nepref, const uint64_t* totals,
+                              const uint64_t* netot, const uint64_t* chunk_base, uint64_t* ttot, uint64_t* tnz,
+                              uint32_t ntexts, uint32_t nslot, uint64_t nchunks) {
+  const uint64_t i = uint64_t(blockIdx.x) * blockDim.x + threadIdx.x;
+  if (i >= uint64_t(ntexts) * nslot) return;
+  const uint32_t t = uint32_t(i / nslot), slot = uint32_t(i % nslot);
+  const uint64_t c0 = chunk_base[t], c1 = chunk_base[t + 1];
+  const uint64_t p0 = c0 < nchunks ? pref[c0 * nslot + slot] : totals[slot];
+  const uint64_t p1 = c1 < nchunks ? pref[c1 * nslot + slot] : totals[slot];
+  const uint64_t z0 = c0 < nchunks ? nepref[c0 * nslot + slot] : netot[slot];
+  const uint64_t z1 = c1 < nchunks ? nepref[c1 * nslot + slot] : netot[slot];
+  ttot[i] = p1 - p0;
+  tnz[i] = z1 - z0;
+}
+
+// Bucket starts of every (text, sub-level); grid (ntexts, jmax), 64 threads.
+__global__ __launch_bounds__(64) void k_bucket_starts_batch(const uint64_t* ttot, const uint64_t* tnz,
+                                                            uint64_t* gstart, uint64_t* nebase, uint64_t* level_ne,
+                                                            uint32_t nslot, int tree) {
+  const uint32_t t = blockIdx.x, j = blockIdx.y + 1;
+  if (threadIdx.x != 0) return;
+  const uint32_t nb = 1u << j, slot0 = nb - 2;
+  const uint64_t* tt = ttot + uint64_t(t) * nslot + slot0;
+  const uint64_t* tz = tnz + uint64_t(t) * nslot + slot0;
+  uint64_t run = 0, runz = 0;
+  for (uint32_t q = 0; q < nb; ++q) {  // q = position in output order
+    const uint32_t b = tree ? rev_bits(q, j) : q;
+    gstart[uint64_t(t) * nslot + slot0 + b] = run;
+    nebase[uint64_t(t) * nslot + slot0 + b] = runz;
+    run += tt[b];
+    runz += tz[b];
+  }
+  level_ne[uint64_t(t) * 8 + j] = runz;
+}
+
+__global__ void k_apply_runs_batch(const uint32_t* counts, const uint64_t* pref, const uint64_t* nepref,
+                                   const uint64_t* chunk_base, const uint64_t* gstart, const uint64_t* nebase,
+                                   uint64_t* tab_dst, uint64_t* tab_src, uint32_t ntexts, uint32_t nslot,
+                                   uint64_t nchunks) {
+  const uint64_t i = uint64_t(blockIdx.x) * blockDim.x + threadIdx.x;
+  if (i >= nchunks * nslot) return;
+  const uint32_t x = counts[i];
+  if (!x) return;
+  const uint64_t c = i / nslot;
+  const uint32_t slot = uint32_t(i - c * nslot);
+  uint32_t lo = 0, hi = ntexts;
+  while (hi - lo > 1) {
+    const uint32_t mid = (lo + hi) >> 1;
+    if (chunk_base[mid] <= c) lo = mid;
+    else hi = mid;
+  }
+  const uint32_t t = lo;
+  const uint64_t c0 = chunk_base[t];
+  const uint64_t pin = pref[i] - pref[c0 * nslot + slot];
+  const uint64_t zin = nepref[i] - nepref[c0 * nslot + slot];
+  const uint32_t j = slot_level(slot);
+  const uint32_t s0 = (1u << j) - 2;
+  uint64_t loc = 0;
+  for (uint32_t b = s0; b < slot; ++b) loc += counts[c * nslot + b];
+  const uint64_t e = uint64_t(s0) * nchunks + c0 * (1ull << j) + nebase[uint64_t(t) * nslot + slot] + zin;
+  tab_dst[e] = gstart[uint64_t(t) * nslot + slot] + pin;
+  tab_src[e] = c * kChunkSyms + loc;
+}
+
+__global__ void k_fill_u32(unsigned int* p, uint32_t n, uint32_t v) {
+  const uint32_t i = blockIdx.x * blockDim.x + threadIdx.x;
+  if (i < n) p[i] = v;
 }
 
 }  // namespace
@@ -1098,6 +1279,195 @@ void build_plain_device(pwmhip_ctx* ctx, const void* d_text, uint64_t n, uint32_
   if (tree && multi) wm_to_wt_device(ctx, d_text, n, width, L, ws);
   ws->n_zeros = tree ? 0 : L;
   for (uint32_t l = 0; l < L; ++l) ws->zeros[l] = tree ? 0 : ctx->pinned[l];
+}
+
+// ---------------------------------------------------------------------------
+// batched construction: many texts, one launch per phase
+// ---------------------------------------------------------------------------
+void build_batch_device(pwmhip_ctx* ctx, const void* const* d_texts, const uint64_t* sizes, uint64_t T,
+                        uint32_t width, uint32_t L, bool tree, bool check_range, pwmhip_ws* outs) {
+  hipStream_t s = ctx->stream;
+  const uint32_t g = L;  // one pass (L <= kMaxGroup)
+  const uint32_t jmax = g - 1;
+  const uint32_t nslot = jmax ? (2u << jmax) - 2u : 0;
+  std::vector<uint64_t> cb(T + 1), out0(T), stride(T), wave_base(T + 1);
+  std::vector<uint64_t> tptr(T), tn(T);
+  uint64_t words = 0, wv = 0;
+  bool aligned = true;
+  const uint64_t per_wave = 64ull * kGatherWindows;
+  for (uint64_t t = 0; t < T; ++t) {
+    cb[t + 1] = cb[t] + (sizes[t] + kChunkSyms - 1) / kChunkSyms;
+    stride[t] = level_stride(sizes[t]);
+    out0[t] = words;
+    words += sizes[t] ? stride[t] * L : 0;
+    wave_base[t] = wv;
+    wv += (words_for(sizes[t]) + per_wave - 1) / per_wave;
+    tptr[t] = reinterpret_cast<uint64_t>(d_texts[t]);
+    tn[t] = sizes[t];
+    if (tptr[t] & 15) aligned = false;
+  }
+  wave_base[T] = wv;
+  const uint64_t C = cb[T];
+  // one allocation for all structures, owned by the first
+  void* block = nullptr;
+  PWM_HIP_TRY(hipMalloc(&block, std::max<uint64_t>(words, 32) * 8));
+  struct BlockGuard {  // frees the block unless the build completes
+    void* p;
+    pwmhip_ws* outs;
+    uint64_t T;
+    ~BlockGuard() {
+      if (!p) return;
+      (void)hipFree(p);
+      for (uint64_t t = 0; t < T; ++t) std::memset(&outs[t], 0, sizeof(pwmhip_ws));
+    }
+  } guard{block, outs, T};
+  for (uint64_t t = 0; t < T; ++t) {
+    pwmhip_ws& ws = outs[t];
+    ws.levels = sizes[t] ? L : 0;
+    ws.text_size = sizes[t];
+    ws.is_tree = tree;
+    ws.d_words = static_cast<uint64_t*>(block) + out0[t];
+    for (uint32_t l = 0; l < L; ++l) {
+      ws.level_offset[l] = l * stride[t];
+      ws.level_bits[l] = sizes[t];
+    }
+    ws.alloc_bytes = 0;
+  }
+  outs[0].d_words = static_cast<uint64_t*>(block);
+  outs[0].alloc_bytes = std::max<uint64_t>(words, 32) * 8;
+  if (C == 0) {
+    guard.p = nullptr;
+    return;
+  }
+
+  const uint64_t tstride = C * kChunkWords + 64;
+  const uint64_t ntiles = (C + kTile - 1) / kTile;
+  const size_t sz_cnt = std::max<size_t>(1, C * nslot);
+  arena sizing;
+  sizing.cap = ~size_t(0) >> 1;
+  auto carve = [&](arena& A, std::vector<void*>& v) {
+    v.clear();
+    v.push_back(A.take<uint64_t>(T + 1));                 // 0 chunk_base
+    v.push_back(A.take<uint64_t>(T));                     // 1 text ptr
+    v.push_back(A.take<uint64_t>(T));                     // 2 n
+    v.push_back(A.take<uint64_t>(T));                     // 3 out0
+    v.push_back(A.take<uint64_t>(T));                     // 4 stride
+    v.push_back(A.take<uint64_t>(T + 1));                 // 5 wave_base
+    v.push_back(A.take<uint64_t>(T * 64));                // 6 zeros
+    v.push_back(A.take<uint64_t>(std::max<size_t>(1, size_t(L - 1)) * tstride));  // 7 temp
+    v.push_back(A.take<uint32_t>(sz_cnt));                // 8 counts
+    v.push_back(A.take<uint64_t>(sz_cnt));                // 9 pref
+    v.push_back(A.take<uint64_t>(sz_cnt));                // 10 nepref
+    v.push_back(A.take<uint64_t>(std::max<size_t>(1, ntiles * nslot)));  // 11 tiles
+    v.push_back(A.take<uint64_t>(std::max<uint32_t>(nslot, 1)));          // 12 totals
+    v.push_back(A.take<uint64_t>(std::max<uint32_t>(nslot, 1)));          // 13 netot
+    v.push_back(A.take<uint64_t>(std::max<size_t>(1, T * nslot)));        // 14 ttot
+    v.push_back(A.take<uint64_t>(std::max<size_t>(1, T * nslot)));        // 15 tnz
+    v.push_back(A.take<uint64_t>(std::max<size_t>(1, T * nslot)));        // 16 gstart
+    v.push_back(A.take<uint64_t>(std::max<size_t>(1, T * nslot)));        // 17 nebase
+    v.push_back(A.take<uint64_t>(T * 8));                 // 18 level_ne
+    v.push_back(A.take<uint64_t>(sz_cnt));                // 19 tab_dst
+    v.push_back(A.take<uint64_t>(sz_cnt));                // 20 tab_src
+    v.push_back(A.take<unsigned long long>(1));           // 21 sym_or
+    v.push_back(A.take<unsigned int>(1));                 // 22 work counter
+  };
+  std::vector<void*> b;
+  carve(sizing, b);
+  ctx->scratch.reset();
+  ctx->scratch.ensure(sizing.used);
+  carve(ctx->scratch, b);
+  auto U64 = [&](int i) { return static_cast<uint64_t*>(b[i]); };
+  PWM_HIP_TRY(hipMemcpyAsync(b[0], cb.data(), (T + 1) * 8, hipMemcpyHostToDevice, s));
+  PWM_HIP_TRY(hipMemcpyAsync(b[1], tptr.data(), T * 8, hipMemcpyHostToDevice, s));
+  PWM_HIP_TRY(hipMemcpyAsync(b[2], tn.data(), T * 8, hipMemcpyHostToDevice, s));
+  PWM_HIP_TRY(hipMemcpyAsync(b[3], out0.data(), T * 8, hipMemcpyHostToDevice, s));
+  PWM_HIP_TRY(hipMemcpyAsync(b[4], stride.data(), T * 8, hipMemcpyHostToDevice, s));
+  PWM_HIP_TRY(hipMemcpyAsync(b[5], wave_base.data(), (T + 1) * 8, hipMemcpyHostToDevice, s));
+  PWM_HIP_TRY(hipMemsetAsync(b[6], 0, T * 64 * 8, s));
+  PWM_HIP_TRY(hipMemsetAsync(b[21], 0, 8, s));
+  hipLaunchKernelGGL(k_fill_u32, dim3(1), dim3(64), 0, s, static_cast<unsigned int*>(b[22]), 1u, 0u);
+
+  PWM_HIP_TRY(hipEventRecord(ctx->ev[0], s));
+  K1Params k1{};
+  k1.width = width;
+  k1.n = 0;
+  k1.L = L;
+  k1.l0 = 0;
+  k1.g = g;
+  k1.np = L;
+  k1.out_level0 = static_cast<uint64_t*>(block);
+  k1.temp_levels = U64(7);
+  k1.temp_payload = U64(7);
+  k1.temp_stride = tstride;
+  k1.counts = static_cast<uint32_t*>(b[8]);
+  k1.nslot = nslot;
+  k1.sym_or = static_cast<unsigned long long*>(b[21]);
+  k1.nchunks = C;
+  k1.aligned16 = aligned ? 1 : 0;
+  k1.b_chunk_base = U64(0);
+  k1.b_text = U64(1);
+  k1.b_n = U64(2);
+  k1.b_out0 = U64(3);
+  k1.b_zeros = static_cast<unsigned long long*>(b[6]);
+  k1.ntexts = uint32_t(T);
+  k1.work = static_cast<unsigned int*>(b[22]);
+  dispatch_k1(ctx, k1);
+  PWM_HIP_TRY(hipEventRecord(ctx->ev[1], s));
+  if (nslot) {
+    const uint32_t* counts = static_cast<const uint32_t*>(b[8]);
+    launch_scan_slots(s, counts, U64(9), U64(12), C, nslot, U64(11), false);
+    launch_scan_slots(s, counts, U64(10), U64(13), C, nslot, U64(11), true);
+    const uint64_t tsl = T * nslot;
+    hipLaunchKernelGGL(k_text_totals, dim3(uint32_t((tsl + 255) / 256)), dim3(256), 0, s, U64(9), U64(10), U64(12),
+                       U64(13), U64(0), U64(14), U64(15), uint32_t(T), nslot, C);
+    hipLaunchKernelGGL(k_bucket_starts_batch, dim3(uint32_t(T), jmax), dim3(64), 0, s, U64(14), U64(15), U64(16),
+                       U64(17), U64(18), nslot, tree ? 1 : 0);
+    const uint64_t nth = C * nslot;
+    hipLaunchKernelGGL(k_apply_runs_batch, dim3(uint32_t((nth + 255) / 256)), dim3(256), 0, s, counts, U64(9),
+                       U64(10), U64(0), U64(16), U64(17), U64(19), U64(20), uint32_t(T), nslot, C);
+    PWM_HIP_TRY(hipEventRecord(ctx->ev[2], s));
+    for (uint32_t j = 1; j <= jmax; ++j) {
+      BatchGatherParams bp{};
+      bp.wave_base = U64(5);
+      bp.chunk_base = U64(0);
+      bp.text_n = U64(2);
+      bp.out0 = U64(3);
+      bp.stride = U64(4);
+      bp.level_ne = U64(18);
+      bp.tab_dst = U64(19);
+      bp.tab_src = U64(20);
+      bp.src = U64(7);
+      bp.src_stride = tstride;
+      bp.out = static_cast<uint64_t*>(block);
+      bp.ntexts = uint32_t(T);
+      bp.j = j;
+      bp.nchunks = C;
+      hipLaunchKernelGGL(k_gather_batch, dim3(uint32_t((wv + 3) / 4)), dim3(256), 0, s, bp);
+    }
+  } else {
+    PWM_HIP_TRY(hipEventRecord(ctx->ev[2], s));
+  }
+  PWM_HIP_TRY(hipGetLastError());
+  PWM_HIP_TRY(hipEventRecord(ctx->ev[3], s));
+  std::vector<uint64_t> zeros(T * 64);
+  uint64_t sym_or = 0;
+  PWM_HIP_TRY(hipMemcpyAsync(zeros.data(), b[6], T * 64 * 8, hipMemcpyDeviceToHost, s));
+  PWM_HIP_TRY(hipMemcpyAsync(&sym_or, b[21], 8, hipMemcpyDeviceToHost, s));
+  PWM_HIP_TRY(hipStreamSynchronize(s));
+  float ms = 0;
+  for (int i = 0; i < 3; ++i) {
+    PWM_HIP_TRY(hipEventElapsedTime(&ms, ctx->ev[i], ctx->ev[i + 1]));
+    ctx->last_ms[i] = ms;
+  }
+  PWM_HIP_TRY(hipEventElapsedTime(&ms, ctx->ev[0], ctx->ev[3]));
+  ctx->last_ms[3] = ms;
+  if (check_range && L < 64 && (sym_or >> L) != 0)
+    throw status_error{PWMHIP_ERR_SYMBOL_RANGE, "symbol >= 2^levels"};
+  for (uint64_t t = 0; t < T; ++t) {
+    outs[t].n_zeros = (tree || !sizes[t]) ? 0 : L;
+    for (uint32_t l = 0; l < L; ++l) outs[t].zeros[l] = (tree || !sizes[t]) ? 0 : zeros[t * 64 + l];
+  }
+  guard.p = nullptr;
 }
 
 }  // namespace pwm

@@ -188,10 +188,64 @@ int pwmhip_build(pwmhip_ctx* ctx, const void* text, uint64_t size, uint32_t word
   });
 }
 
+int pwmhip_build_batch(pwmhip_ctx* ctx, const void* const* texts, const uint64_t* sizes, uint64_t count,
+                       uint32_t word_width, uint32_t levels, uint32_t algo, uint32_t flags, pwmhip_ws* outs) {
+  if (!ctx || (count && (!texts || !sizes || !outs))) return PWMHIP_ERR_INVALID;
+  if (flags & PWMHIP_FLAG_REUSE_OUTPUT) return PWMHIP_ERR_INVALID;
+  for (uint64_t t = 0; t < count; ++t) std::memset(&outs[t], 0, sizeof(pwmhip_ws));
+  if (word_width != 1 && word_width != 2 && word_width != 4 && word_width != 8) return PWMHIP_ERR_INVALID;
+  if (algo > PWMHIP_ALGO_HUFF_DD_PC) return PWMHIP_ERR_INVALID;
+  const bool huff = pwm::is_huffman_algo(algo);
+  const bool tree = (flags & PWMHIP_FLAG_TREE) != 0;
+  if (!huff && (levels == 0 || levels > 8 * word_width || levels > PWMHIP_MAX_LEVELS)) return PWMHIP_ERR_INVALID;
+  for (uint64_t t = 0; t < count; ++t)
+    if (sizes[t] > 0 && !texts[t]) return PWMHIP_ERR_INVALID;
+  if (count == 0) return PWMHIP_OK;
+  if (huff || levels > uint32_t(pwm::kMaxGroup)) {
+    // shapes the batched kernels do not cover: one build per text, each
+    // structure owning its memory
+    for (uint64_t t = 0; t < count; ++t) {
+      const int st = pwmhip_build(ctx, texts[t], sizes[t], word_width, levels, algo,
+                                  flags & ~uint32_t(PWMHIP_FLAG_REUSE_OUTPUT), &outs[t]);
+      if (st != PWMHIP_OK) {
+        for (uint64_t u = 0; u < t; ++u) pwmhip_ws_free(ctx, &outs[u]);
+        return st;
+      }
+    }
+    return PWMHIP_OK;
+  }
+  return pwm::run_guarded("build_batch", [&] {
+    PWM_HIP_TRY(hipSetDevice(ctx->device));
+    std::vector<const void*> d_texts(texts, texts + count);
+    if (!(flags & PWMHIP_FLAG_TEXT_ON_DEVICE)) {
+      // stage every host text at a 256-byte aligned offset of one buffer
+      std::vector<uint64_t> off(count);
+      uint64_t total = 0;
+      for (uint64_t t = 0; t < count; ++t) {
+        off[t] = total;
+        total += (sizes[t] * word_width + 255) & ~uint64_t(255);
+      }
+      ctx->text_scratch.ensure(total + 64);
+      for (uint64_t t = 0; t < count; ++t) {
+        uint8_t* d = static_cast<uint8_t*>(ctx->text_scratch.base) + off[t];
+        if (sizes[t])
+          PWM_HIP_TRY(hipMemcpyAsync(d, texts[t], sizes[t] * word_width, hipMemcpyHostToDevice, ctx->stream));
+        d_texts[t] = d;
+      }
+    }
+    pwm::build_batch_device(ctx, d_texts.data(), sizes, count, word_width, levels, tree,
+                            (flags & PWMHIP_FLAG_CHECK_RANGE) != 0, outs);
+    if (!tree && pwm::zeros_last_is_zero(algo))
+      for (uint64_t t = 0; t < count; ++t)
+        if (outs[t].levels) outs[t].zeros[levels - 1] = 0;
+  });
+}
+
 int pwmhip_ws_free(pwmhip_ctx* ctx, pwmhip_ws* ws) {
   if (!ctx || !ws) return PWMHIP_ERR_INVALID;
   (void)hipSetDevice(ctx->device);
-  if (ws->d_words) (void)hipFree(ws->d_words);
+  // a structure with alloc_bytes == 0 is a view into a batch's memory
+  if (ws->d_words && ws->alloc_bytes) (void)hipFree(ws->d_words);
   std::free(ws->code_len);
   std::free(ws->code_word);
   std::memset(ws, 0, sizeof(*ws));

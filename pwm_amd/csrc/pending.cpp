@@ -4,6 +4,4 @@ namespace pwm {
 }  // namespace pwm
 extern "C" {
 int pwmhip_reduce_alphabet(pwmhip_ctx*, void*, uint64_t, uint32_t, uint64_t*) { return PWMHIP_ERR_UNSUPPORTED; }
-int pwmhip_build_batch(pwmhip_ctx*, const void* const*, const uint64_t*, uint64_t, uint32_t, uint32_t, uint32_t,
-                       uint32_t, pwmhip_ws*) { return PWMHIP_ERR_UNSUPPORTED; }
 }

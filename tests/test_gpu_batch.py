@@ -1,0 +1,108 @@
+"""Batched construction (pwmhip_build_batch): every structure of a batch is
+bit-identical to the single-text build of its text and to the oracle --
+uneven and empty texts, chunk-boundary sizes, every plain shape, trees and
+matrices, host and device texts."""
+import numpy as np
+import pytest
+
+import oracle_lib as O
+
+pwm_amd = pytest.importorskip("pwm_amd")
+
+DT = {1: np.uint8, 2: np.uint16, 4: np.uint32, 8: np.uint64}
+
+
+def _same(a, b, key):
+    assert a.levels() == b.levels(), key
+    assert a.text_size() == b.text_size(), key
+    for l, (x, y) in enumerate(zip(a.bvs(), b.bvs())):
+        assert np.array_equal(x, y), (key, l)
+    assert np.array_equal(a.zeros(), b.zeros()), key
+
+
+def _texts(rng, sizes, w, L):
+    return [rng.integers(0, 1 << L, n, dtype=np.uint64).astype(DT[w]) for n in sizes]
+
+
+EDGE_SIZES = [0, 1, 63, 64, 65, 4095, 4096, 4097, 0, 8191, 12289, 3, 100000, 1, 0, 77777]
+
+
+@pytest.mark.gpu
+@pytest.mark.parametrize("w,L", [(1, 1), (1, 2), (1, 3), (1, 5), (1, 8), (2, 4), (2, 7), (4, 8), (8, 6)])
+def test_batch_equals_single(w, L):
+    rng = np.random.default_rng(31 * w + L)
+    texts = _texts(rng, EDGE_SIZES, w, L)
+    for tree in (False, True):
+        for algo in ("ppc", "pps"):
+            got = pwm_amd.build_batch(texts, L, algo, tree, check_range=True)
+            assert len(got) == len(texts)
+            for i, t in enumerate(texts):
+                ref = pwm_amd.build(t, L, algo, tree)
+                _same(got[i], ref, (w, L, tree, algo, i, t.size))
+                if t.size and t.size < 20000:
+                    o = O.naive(t, L, tree)
+                    for l, (x, y) in enumerate(zip(got[i].bvs(), o.levels)):
+                        assert np.array_equal(x, y), (w, L, tree, i, l)
+                ref.free()
+
+
+@pytest.mark.gpu
+def test_batch_many_uneven_dna_texts():
+    """2000 DNA texts, log-normal lengths (median 4 KiB) -- the shape of the
+    many-texts workload at a size the check finishes quickly."""
+    rng = np.random.default_rng(2024)
+    sizes = np.clip(rng.lognormal(np.log(4096), 1.5, 2000), 1, 1 << 20).astype(np.int64)
+    sizes[::97] = 0
+    big = pwm_amd.generate("dna", 0xBA7C, int(sizes.sum()) + 1)
+    offs = np.concatenate([[0], np.cumsum(sizes)])
+    texts = [big[offs[i]:offs[i + 1]] for i in range(len(sizes))]
+    got = pwm_amd.build_batch(texts, 3, "ppc", False)
+    sel = list(range(0, 2000, 37)) + [int(np.argmax(sizes))]
+    for i in sel:
+        ref = pwm_amd.build(texts[i], 3, "ppc", False)
+        _same(got[i], ref, i)
+        ref.free()
+    # zero counts of all texts against the oracle's definition
+    for i in range(0, 2000, 11):
+        t = texts[i]
+        if t.size:
+            z = got[i].zeros()
+            assert int(z[0]) == int(np.count_nonzero((t >> 2) & 1 == 0))
+            assert int(z[2]) == 0  # ppc leaves zeros[L-1] at 0
+
+
+@pytest.mark.gpu
+def test_batch_device_texts_and_lifetime():
+    import torch
+    rng = np.random.default_rng(9)
+    texts = _texts(rng, [5000, 70000, 1, 4096 * 3], 1, 8)
+    dev = [torch.from_numpy(t).to("cuda:0") for t in texts]
+    got = pwm_amd.build_batch(dev, 8, "dd_pc", True)
+    first = got[0]
+    del got[1:]  # the others' memory stays valid while one structure lives
+    o = O.naive(texts[0], 8, True)
+    for x, y in zip(first.bvs(), o.levels):
+        assert np.array_equal(x, y)
+    first.free()
+
+
+@pytest.mark.gpu
+def test_batch_fallback_shapes_and_errors():
+    rng = np.random.default_rng(4)
+    # more than 8 levels and Huffman shapes build text by text
+    texts = _texts(rng, [3000, 0, 9000], 2, 12)
+    got = pwm_amd.build_batch(texts, 12, "pc", False)
+    for i, t in enumerate(texts):
+        ref = pwm_amd.build(t, 12, "pc", False)
+        _same(got[i], ref, ("L12", i))
+    dna = [pwm_amd.generate("dna", 5 + i, 20000 + 17 * i) for i in range(3)]
+    got = pwm_amd.build_batch(dna, 0, "huff_naive", True)
+    for i, t in enumerate(dna):
+        ref = pwm_amd.build(t, 0, "huff_naive", True)
+        _same(got[i], ref, ("huff", i))
+        assert np.array_equal(got[i].codes()[0], ref.codes()[0])
+    with pytest.raises(pwm_amd.PwmError) as e:
+        pwm_amd.build_batch([np.array([1, 2], np.uint8), np.array([9], np.uint8)], 3, "ppc", check_range=True)
+    assert e.value.status == pwm_amd.ERR_SYMBOL_RANGE
+    with pytest.raises(pwm_amd.PwmError):
+        pwm_amd.build_batch([np.zeros(3, np.uint8), np.zeros(3, np.uint16)], 3, "ppc")
