@@ -37,6 +37,13 @@ CONFIGS = {
 }
 
 
+# Many uneven texts, built together: (seed, texts, median length, sigma,
+# min length, max length, levels, algo, tree)
+BATCH_CONFIGS = {
+    "dna_batch20k": (0xC4, 20000, 65536, 1.5, 1024, 64 << 20, 3, "ppc", False),
+}
+
+
 def cpu_baseline(text, levels, tree, width, sample_bytes):
     """Reference algorithms (wx_ppc / wx_pps / wx_dd_pc) on a prefix sample."""
     sys.path.insert(0, os.path.join(ROOT, "tests"))
@@ -73,12 +80,164 @@ def traffic_from_profiles(config):
     return None
 
 
+def batch_sizes(seed, T, median, sigma, lo, hi):
+    rng = np.random.default_rng(seed)
+    return np.clip(rng.lognormal(np.log(median), sigma, T), lo, hi).astype(np.int64)
+
+
+def cpu_baseline_batch(host, offs, sizes, levels, tree, sample_bytes):
+    """Reference algorithms text by text (one call per text, as the reference
+    builds) over the first texts of the batch up to `sample_bytes`."""
+    sys.path.insert(0, os.path.join(ROOT, "tests"))
+    import oracle_lib
+    if not oracle_lib.have_ref():
+        return None
+    cores = len(os.sched_getaffinity(0))
+    os.environ.setdefault("OMP_NUM_THREADS", str(cores))
+    k = int(np.searchsorted(np.cumsum(sizes), sample_bytes)) + 1
+    k = min(k, len(sizes))
+    sample = [host[offs[i]:offs[i] + sizes[i]] for i in range(k)]
+    nbytes = int(sum(x.size for x in sample))
+    per, best = {}, None
+    for algo in ("ppc", "pps", "dd_pc"):
+        oracle_lib.ref_build(algo, sample[0], levels, tree)
+        t = sum(oracle_lib.ref_build(algo, x, levels, tree)[1] for x in sample)
+        per[algo] = round(nbytes / t / 1e9, 3)
+        if best is None or per[algo] > best[1]:
+            best = (algo, per[algo])
+    return {"value": best[1], "unit": "GB/s", "cores": int(os.environ["OMP_NUM_THREADS"]), "kind": "reference",
+            "sample": "first %d texts of the batch (%d MiB), one reference call per text; wx_%s best of "
+                      "ppc/pps/dd_pc: %s" % (k, nbytes >> 20, best[0], per)}
+
+
+def run_batch(args):
+    """Batched construction of many uneven texts against one build per text."""
+    import ctypes
+    import torch
+    import pwm_amd
+
+    seed, T, median, sigma, lo, hi, levels, algo, tree = BATCH_CONFIGS[args.config]
+    sizes = batch_sizes(seed, T, median, sigma, lo, hi)
+    padded = (sizes + 255) // 256 * 256
+    offs = np.concatenate([[0], np.cumsum(padded)[:-1]]).astype(np.int64)
+    total = int(sizes.sum())
+    t0 = time.perf_counter()
+    host = pwm_amd.generate("dna", seed, int(padded.sum()))  # texts sit at 256-byte aligned offsets
+    gen_s = time.perf_counter() - t0
+    d_all = torch.from_numpy(host).to("cuda:0")
+    torch.cuda.synchronize()
+    lib = pwm_amd.lib()
+    ctx = pwm_amd.Context(0)
+    base = d_all.data_ptr()
+    ptrs = (ctypes.c_void_p * T)(*[base + int(o) for o in offs])
+    csz = (ctypes.c_uint64 * T)(*[int(x) for x in sizes])
+    flags = pwm_amd.FLAG_TEXT_ON_DEVICE | (pwm_amd.FLAG_TREE if tree else 0)
+    code = pwm_amd.ALGOS[algo]
+    outs = (pwm_amd._WS * T)()
+
+    def batched():
+        st = lib.pwmhip_build_batch(ctx.h, ptrs, csz, T, 1, levels, code, flags | pwm_amd.FLAG_REUSE_OUTPUT, outs)
+        if st != pwm_amd.OK:
+            raise pwm_amd.PwmError(st, "build_batch")
+
+    for _ in range(args.warmup):
+        batched()
+    torch.cuda.synchronize()
+    dev = []
+    t0 = time.perf_counter()
+    for _ in range(args.steps):
+        batched()
+        dev.append(ctx.last_timing())
+    torch.cuda.synchronize()
+    batch_ms = (time.perf_counter() - t0) / args.steps * 1e3
+    dev_ms = float(np.mean([d[3] for d in dev]))
+    phases = [round(float(np.mean([d[i] for d in dev])), 4) for i in range(3)]
+
+    # one plain build per text, each into its own reused output
+    singles = (pwm_amd._WS * T)()
+    sflags = flags | pwm_amd.FLAG_REUSE_OUTPUT
+
+    def per_text():
+        devsum = 0.0
+        arr = (ctypes.c_double * 4)()
+        for i in range(T):
+            st = lib.pwmhip_build(ctx.h, ptrs[i], csz[i], 1, levels, code, sflags, ctypes.byref(singles[i]))
+            if st != pwm_amd.OK:
+                raise pwm_amd.PwmError(st, "build")
+            lib.pwmhip_last_timing(ctx.h, arr)
+            devsum += arr[3]
+        return devsum
+
+    per_text()  # allocates every output once
+    torch.cuda.synchronize()
+    t0 = time.perf_counter()
+    single_dev_ms = per_text()
+    torch.cuda.synchronize()
+    single_ms = (time.perf_counter() - t0) * 1e3
+
+    # bit-exact: the batched structures equal the per-text ones
+    rng = np.random.default_rng(1)
+    check = sorted(set(rng.integers(0, T, 200).tolist()) | {int(np.argmax(sizes)), int(np.argmin(sizes))})
+    for i in check:
+        a, b = outs[i], singles[i]
+        assert a.levels == b.levels and list(a.zeros[:levels]) == list(b.zeros[:levels]), i
+        for l in range(a.levels):
+            nw = (int(a.level_bits[l]) + 63) // 64
+            x = np.empty(nw, np.uint64)
+            y = np.empty(nw, np.uint64)
+            lib.pwmhip_ws_download(ctx.h, ctypes.byref(a), l, x.ctypes.data_as(ctypes.c_void_p))
+            lib.pwmhip_ws_download(ctx.h, ctypes.byref(b), l, y.ctypes.data_as(ctypes.c_void_p))
+            assert np.array_equal(x, y), (i, l)
+
+    words = ((sizes + 63) // 64).sum()
+    b_min = total + levels * 8 * int(words)
+    achieved = b_min / (dev_ms / 1e3) / 1e9
+    out = {
+        "metric": "WM/WT build input GB/s + % HBM peak; batched vs per-text launches, bit-exact",
+        "value": round(total / (batch_ms / 1e3) / 1e9, 2),
+        "unit": "GB/s",
+        "n_gpus": 1,
+        "steps": args.steps,
+        "warmup": args.warmup,
+        "ms_per_step": round(batch_ms, 4),
+        "higher_is_better": True,
+        "scaling": "weak",
+        "vs_baseline": None,
+        "dtype": "u8",
+        "data": "synthetic (dna generator, seed 0x%X; lognormal lengths)" % seed,
+        "config": {"workload": "%s: %d uint8 DNA texts, lengths lognormal(median %d, sigma %.1f) clipped "
+                               "[%d, %d], %d symbols total, %d levels, %s %s, one batched build vs one "
+                               "build per text" % (args.config, T, median, sigma, lo, hi, total, levels,
+                                                   "wt" if tree else "wm", algo),
+                   "texts": T, "symbols": total, "levels": levels,
+                   "algorithm": ("wt_" if tree else "wm_") + algo,
+                   "device_ms_per_build": round(dev_ms, 4),
+                   "device_ms_phases": {"local_build": phases[0], "offsets": phases[1], "stitch": phases[2]},
+                   "per_text_ms": round(single_ms, 2),
+                   "per_text_device_ms": round(single_dev_ms, 2),
+                   "speedup_vs_per_text": round(single_ms / batch_ms, 2),
+                   "checked_texts": len(check),
+                   "generate_s": round(gen_s, 2)},
+        "roofline": {"bound": "hbm", "achieved": round(achieved, 1), "peak": HBM_PEAK_GBS, "unit": "GB/s",
+                     "frac": round(achieved / HBM_PEAK_GBS, 4),
+                     "traffic": traffic_from_profiles(args.config)},
+    }
+    if not args.no_cpu:
+        out["cpu_baseline"] = cpu_baseline_batch(host, offs, sizes, levels, tree, args.cpu_sample_mib << 20)
+    for i in range(T):
+        lib.pwmhip_ws_free(ctx.h, ctypes.byref(singles[i]))
+    for i in range(T - 1, -1, -1):
+        lib.pwmhip_ws_free(ctx.h, ctypes.byref(outs[i]))
+    ctx.close()
+    print(json.dumps(out), flush=True)
+
+
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
     ap.add_argument("--steps", type=int, default=10)
     ap.add_argument("--warmup", type=int, default=3)
-    ap.add_argument("--config", default="dna4g_wm", choices=sorted(CONFIGS))
+    ap.add_argument("--config", default="dna4g_wm", choices=sorted(CONFIGS) + sorted(BATCH_CONFIGS))
     ap.add_argument("--n", type=int, default=0, help="override the text length")
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--cpu-sample-mib", type=int, default=256)
@@ -87,6 +246,10 @@ def main():
         print("bench.py: this build measures one GPU only; --gpus %d is not supported "
               "(the construction does not shard across GPUs here)" % args.gpus, file=sys.stderr)
         sys.exit(2)
+
+    if args.config in BATCH_CONFIGS:
+        run_batch(args)
+        return
 
     import torch
     import pwm_amd

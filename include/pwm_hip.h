@@ -167,7 +167,13 @@ int pwmhip_reduce_alphabet(pwmhip_ctx* ctx, void* d_text, uint64_t size, uint32_
  * No counterpart in the reference (it builds one text per call).  Builds
  * `count` independent texts (host or device pointers, uniform word width,
  * levels and algorithm) in one persistent launch per phase from a device
- * work queue; results identical to one pwmhip_build per text. */
+ * work queue; results identical to one pwmhip_build per text.
+ * Plain algorithms with levels <= 8: every structure lives in one device
+ * allocation owned by outs[0] (alloc_bytes > 0); outs[1..] are views
+ * (alloc_bytes == 0, pwmhip_ws_free frees nothing of theirs), so free
+ * outs[0] last.  PWMHIP_FLAG_REUSE_OUTPUT takes over outs[0]'s earlier block
+ * when it is large enough.  Other shapes (Huffman, more levels) are built
+ * text by text and every structure owns its memory. */
 int pwmhip_build_batch(pwmhip_ctx* ctx, const void* const* texts, const uint64_t* sizes,
                        uint64_t count, uint32_t word_width, uint32_t levels, uint32_t algo,
                        uint32_t flags, pwmhip_ws* outs);

@@ -142,6 +142,17 @@ __device__ __forceinline__ uint32_t rank1_lds(const uint64_t* plane, const uint3
   return ooff[w] + __popcll(plane[w] & ((1ull << r) - 1));
 }
 
+// Device-memory view of a pointer: global (not flat) loads and stores, so
+// waits for LDS traffic never wait on them.
+template <typename T>
+__device__ __forceinline__ auto gmem(T* p) {
+#if __HIP_DEVICE_COMPILE__
+  return (__attribute__((address_space(1))) T*)p;
+#else
+  return p;  // host pass: never executed
+#endif
+}
+
 template <typename T>
 __device__ __forceinline__ uint64_t ld_sym(const void* text, uint64_t i) {
   return static_cast<const T*>(text)[i];
@@ -163,10 +174,9 @@ struct K1Params {
   unsigned long long* zeros_out;  // zeros of sub-level j accumulate into zeros_out[j]
   unsigned long long* sym_or;
   uint64_t nchunks;
-  int aligned16;             // byte text 16-byte aligned: vector loads + prefetch
   // batch mode (b_chunk_base != nullptr): text t owns chunks
   // [b_chunk_base[t], b_chunk_base[t+1]); its level 0 starts at word
-  // b_out0[t] of out_level0 and its zero counts at b_zeros[t*64]
+  // b_out0[t] of out_level0 and its zero counts at b_zeros[t*kMaxGroup]
   const uint64_t* b_chunk_base;
   const uint64_t* b_text;
   const uint64_t* b_n;
@@ -175,6 +185,9 @@ struct K1Params {
   uint32_t ntexts;
   unsigned int* work;        // dynamic chunk queue (atomic counter), or nullptr
 };
+
+// Chunks a wave takes from the work queue at a time.
+constexpr uint32_t kGrab = 16;
 
 // Where a chunk's symbols come from and where its level-l0 word goes.
 struct ChunkView {
@@ -185,7 +198,10 @@ struct ChunkView {
   unsigned long long* zout;  // zero counters of the chunk's text
 };
 
-__device__ __forceinline__ ChunkView chunk_view(const K1Params& p, uint64_t c) {
+// `hint` is the text of the wave's previous chunk: chunks arrive in
+// increasing order within a grab, so a few forward steps usually find the
+// text; otherwise a binary search over the chunk bases.
+__device__ __forceinline__ ChunkView chunk_view(const K1Params& p, uint64_t c, uint32_t& hint) {
   ChunkView v;
   if (!p.b_chunk_base) {
     v.text = p.text;
@@ -195,18 +211,30 @@ __device__ __forceinline__ ChunkView chunk_view(const K1Params& p, uint64_t c) {
     v.zout = p.zeros_out;
     return v;
   }
-  uint32_t lo = 0, hi = p.ntexts;  // chunk_base[lo] <= c < chunk_base[hi]
-  while (hi - lo > 1) {
-    const uint32_t mid = (lo + hi) >> 1;
-    if (p.b_chunk_base[mid] <= c) lo = mid;
-    else hi = mid;
+  uint32_t lo = hint;
+  if (!(p.b_chunk_base[lo] <= c && c < p.b_chunk_base[lo + 1])) {
+    int steps = 0;
+    while (steps < 4 && lo + 1 < p.ntexts && p.b_chunk_base[lo + 1] <= c) {
+      ++lo;
+      ++steps;
+    }
+    if (!(p.b_chunk_base[lo] <= c && c < p.b_chunk_base[lo + 1])) {
+      lo = 0;
+      uint32_t hi = p.ntexts;  // chunk_base[lo] <= c < chunk_base[hi]
+      while (hi - lo > 1) {
+        const uint32_t mid = (lo + hi) >> 1;
+        if (p.b_chunk_base[mid] <= c) lo = mid;
+        else hi = mid;
+      }
+    }
   }
+  hint = lo;
   const uint64_t lc = c - p.b_chunk_base[lo];
   v.text = reinterpret_cast<const void*>(p.b_text[lo]);
   v.base = lc * kChunkSyms;
   v.n = p.b_n[lo];
   v.lvl0 = p.out_level0 + p.b_out0[lo] + lc * kChunkWords;
-  v.zout = p.b_zeros + uint64_t(lo) * 64;
+  v.zout = p.b_zeros + uint64_t(lo) * kMaxGroup;
   return v;
 }
 
@@ -220,13 +248,13 @@ template <int NP, typename T>
 __device__ __forceinline__ void load_planes_text(const K1Params& p, const void* text, uint64_t e0, int cnt,
                                                  uint64_t* acc, uint64_t& my_or) {
   // symbol bit of plane k: s_k = L-1-l0-k ; byte index s_k>>3, bit s_k&7
-  const T* tx = static_cast<const T*>(text);
+  const auto tx = gmem(static_cast<const T*>(text));
 #pragma unroll 1
   for (int q = 0; q < 16; ++q) {
     uint64_t v[4];
     if (cnt >= 4 * q + 4) {
       if constexpr (sizeof(T) == 1) {
-        const uint32_t x = *reinterpret_cast<const uint32_t*>(tx + e0 + 4 * q);
+        const uint32_t x = *gmem(reinterpret_cast<const uint32_t*>(static_cast<const T*>(text) + e0 + 4 * q));
         v[0] = x & 0xff; v[1] = (x >> 8) & 0xff; v[2] = (x >> 16) & 0xff; v[3] = x >> 24;
       } else {
 #pragma unroll
@@ -348,25 +376,50 @@ __global__ __launch_bounds__(64 * waves_for<NP>()) void k_local_build(K1Params p
   __syncthreads();
   WaveLds<NP>& L = wl[wid];
   uint64_t my_or = 0;
-  uint64_t zsum[kMaxGroup];  // zeros per sub-level over this wave's chunks
+  // zeros per sub-level over this wave's recent chunks (flushed before
+  // 2^19 chunks could overflow 32 bits)
+  constexpr int kZ = NP < kMaxGroup ? NP : kMaxGroup;
+  uint32_t zsum[kZ];
 #pragma unroll
-  for (int j = 0; j < kMaxGroup; ++j) zsum[j] = 0;
+  for (int j = 0; j < kZ; ++j) zsum[j] = 0;
+  uint32_t zchunks = 0;
 
   const uint64_t gw = uint64_t(blockIdx.x) * WAVES + wid;
   const uint64_t tw = uint64_t(gridDim.x) * WAVES;
-  const bool batch = p.b_chunk_base != nullptr;
-  // next chunk of this wave: static round robin, or the device work queue
+  // next chunk of this wave: static round robin, or grabs of kGrab
+  // consecutive chunks from the device work queue
+  uint64_t grab_end = 0;
   auto next_chunk = [&](uint64_t prev, bool first) -> uint64_t {
     if (!p.work) return first ? gw : prev + tw;
-    uint32_t c = 0;
-    if (lane == 0) c = atomicAdd(p.work, 1u);
-    return __builtin_amdgcn_readfirstlane(c);
+    if (!first && prev + 1 < grab_end) return prev + 1;
+    uint32_t q = 0;
+    if (lane == 0) q = atomicAdd(p.work, 1u);
+    q = __builtin_amdgcn_readfirstlane(q);
+    const uint64_t c0 = uint64_t(q) * kGrab;
+    grab_end = min<uint64_t>(c0 + kGrab, p.nchunks);
+    return c0;
   };
-  const bool fast_mode = !p.planes_in && p.width == 1 && p.aligned16;
-  auto full_chunk = [&](const ChunkView& v) { return v.base + kChunkSyms <= v.n; };
+  uint32_t hint = 0;
+  // zero counts accumulate per wave while the chunks belong to one text
+  unsigned long long* zdst = nullptr;
+  auto flush_zeros = [&]() {
+    if (zdst && lane == 0) {
+#pragma unroll
+      for (int j = 0; j < kZ; ++j)
+        if (j < int(p.g) && zsum[j]) atomicAdd(zdst + j, (unsigned long long)zsum[j]);
+    }
+#pragma unroll
+    for (int j = 0; j < kZ; ++j) zsum[j] = 0;
+    zchunks = 0;
+  };
+  // byte texts at 16-byte aligned addresses take 16-byte loads and prefetch
+  const bool fast_mode = !p.planes_in && p.width == 1;
+  auto full_chunk = [&](const ChunkView& v) {
+    return v.base + kChunkSyms <= v.n && (reinterpret_cast<uintptr_t>(v.text) & 15) == 0;
+  };
   auto load_chunk = [&](const ChunkView& v, uint4* d) {
-    const uint4* src =
-        reinterpret_cast<const uint4*>(static_cast<const uint8_t*>(v.text) + v.base + 64ull * lane);
+    const auto src =
+        gmem(reinterpret_cast<const uint4*>(static_cast<const uint8_t*>(v.text) + v.base + 64ull * lane));
     d[0] = src[0];
     d[1] = src[1];
     d[2] = src[2];
@@ -376,7 +429,7 @@ __global__ __launch_bounds__(64 * waves_for<NP>()) void k_local_build(K1Params p
   uint64_t c = next_chunk(0, true);
   ChunkView cv;
   if (c < p.nchunks) {
-    cv = chunk_view(p, c);
+    cv = chunk_view(p, c, hint);
     if (fast_mode && full_chunk(cv)) load_chunk(cv, pf);
   }
 
@@ -384,7 +437,12 @@ __global__ __launch_bounds__(64 * waves_for<NP>()) void k_local_build(K1Params p
     wave_sync();
     const uint64_t cn = next_chunk(c, false);
     ChunkView nv;
-    if (cn < p.nchunks) nv = chunk_view(p, cn);
+    if (cn < p.nchunks) nv = chunk_view(p, cn, hint);
+    if (cv.zout != zdst || zchunks == (1u << 19)) {
+      flush_zeros();
+      zdst = cv.zout;
+    }
+    ++zchunks;
     const bool have_cur = fast_mode && full_chunk(cv);
     const uint32_t Ec = uint32_t(min<uint64_t>(kChunkSyms, cv.n - cv.base));
     const uint32_t nw = (Ec + 63) >> 6;
@@ -429,7 +487,7 @@ __global__ __launch_bounds__(64 * waves_for<NP>()) void k_local_build(K1Params p
     for (uint32_t j = 0; j < p.g; ++j) {
       const uint64_t m = L.buf[cur][j][lane];
       if (lane < int(nw)) {
-        if (j == 0) cv.lvl0[lane] = brev64(m);
+        if (j == 0) gmem(cv.lvl0)[lane] = brev64(m);
         else p.temp_levels[(j - 1) * p.temp_stride + c * P + lane] = brev64(m);
       }
       const uint64_t m1 = m & vm, m0 = ~m & vm;
@@ -438,11 +496,7 @@ __global__ __launch_bounds__(64 * waves_for<NP>()) void k_local_build(K1Params p
       const uint32_t zoff = zinc - z, Zt = __builtin_amdgcn_readlane(zinc, 63);
       const uint32_t oof = min(64u * lane, Ec) - zoff;  // every earlier word is full
       const uint32_t Ot = Ec - Zt;
-      if (batch) {
-        if (lane == 0 && Zt) atomicAdd(cv.zout + j, (unsigned long long)Zt);
-      } else {
-        zsum[j] += Zt;
-      }
+      zsum[j] += Zt;
       const bool more = (j + 1 < p.g) || (p.np > p.g);
       if (!more) break;
       L.ooff[lane] = oof;
@@ -497,11 +551,7 @@ __global__ __launch_bounds__(64 * waves_for<NP>()) void k_local_build(K1Params p
     c = cn;
     cv = nv;
   }
-  if (lane == 0 && !batch) {
-#pragma unroll
-    for (int j = 0; j < kMaxGroup; ++j)
-      if (j < int(p.g) && zsum[j]) atomicAdd(p.zeros_out + j, (unsigned long long)zsum[j]);
-  }
+  flush_zeros();
   if (!p.planes_in) {
 #pragma unroll
     for (int d = 32; d >= 1; d >>= 1) my_or |= __shfl_xor(my_or, d, 64);
@@ -1201,7 +1251,6 @@ void build_plain_device(pwmhip_ctx* ctx, const void* d_text, uint64_t n, uint32_
     k1.zeros_out = reinterpret_cast<unsigned long long*>(zeros_dev + l0);
     k1.sym_or = sym_or;
     k1.nchunks = nchunks;
-    k1.aligned16 = (reinterpret_cast<uintptr_t>(d_text) & 15) == 0;
     dispatch_k1(ctx, k1);
     if (gi == 0) PWM_HIP_TRY(hipEventRecord(ctx->ev[1], s));
 
@@ -1285,7 +1334,8 @@ void build_plain_device(pwmhip_ctx* ctx, const void* d_text, uint64_t n, uint32_
 // batched construction: many texts, one launch per phase
 // ---------------------------------------------------------------------------
 void build_batch_device(pwmhip_ctx* ctx, const void* const* d_texts, const uint64_t* sizes, uint64_t T,
-                        uint32_t width, uint32_t L, bool tree, bool check_range, pwmhip_ws* outs) {
+                        uint32_t width, uint32_t L, bool tree, bool check_range, pwmhip_ws* outs, void* reuse,
+                        uint64_t reuse_bytes) {
   hipStream_t s = ctx->stream;
   const uint32_t g = L;  // one pass (L <= kMaxGroup)
   const uint32_t jmax = g - 1;
@@ -1293,7 +1343,6 @@ void build_batch_device(pwmhip_ctx* ctx, const void* const* d_texts, const uint6
   std::vector<uint64_t> cb(T + 1), out0(T), stride(T), wave_base(T + 1);
   std::vector<uint64_t> tptr(T), tn(T);
   uint64_t words = 0, wv = 0;
-  bool aligned = true;
   const uint64_t per_wave = 64ull * kGatherWindows;
   for (uint64_t t = 0; t < T; ++t) {
     cb[t + 1] = cb[t] + (sizes[t] + kChunkSyms - 1) / kChunkSyms;
@@ -1304,13 +1353,20 @@ void build_batch_device(pwmhip_ctx* ctx, const void* const* d_texts, const uint6
     wv += (words_for(sizes[t]) + per_wave - 1) / per_wave;
     tptr[t] = reinterpret_cast<uint64_t>(d_texts[t]);
     tn[t] = sizes[t];
-    if (tptr[t] & 15) aligned = false;
   }
   wave_base[T] = wv;
   const uint64_t C = cb[T];
   // one allocation for all structures, owned by the first
+  const uint64_t block_bytes = std::max<uint64_t>(words, 32) * 8;
   void* block = nullptr;
-  PWM_HIP_TRY(hipMalloc(&block, std::max<uint64_t>(words, 32) * 8));
+  uint64_t owned_bytes = block_bytes;
+  if (reuse && reuse_bytes >= block_bytes) {
+    block = reuse;
+    owned_bytes = reuse_bytes;
+  } else {
+    if (reuse) (void)hipFree(reuse);
+    PWM_HIP_TRY(hipMalloc(&block, block_bytes));
+  }
   struct BlockGuard {  // frees the block unless the build completes
     void* p;
     pwmhip_ws* outs;
@@ -1334,7 +1390,7 @@ void build_batch_device(pwmhip_ctx* ctx, const void* const* d_texts, const uint6
     ws.alloc_bytes = 0;
   }
   outs[0].d_words = static_cast<uint64_t*>(block);
-  outs[0].alloc_bytes = std::max<uint64_t>(words, 32) * 8;
+  outs[0].alloc_bytes = owned_bytes;
   if (C == 0) {
     guard.p = nullptr;
     return;
@@ -1353,7 +1409,7 @@ void build_batch_device(pwmhip_ctx* ctx, const void* const* d_texts, const uint6
     v.push_back(A.take<uint64_t>(T));                     // 3 out0
     v.push_back(A.take<uint64_t>(T));                     // 4 stride
     v.push_back(A.take<uint64_t>(T + 1));                 // 5 wave_base
-    v.push_back(A.take<uint64_t>(T * 64));                // 6 zeros
+    v.push_back(A.take<uint64_t>(T * kMaxGroup));         // 6 zeros
     v.push_back(A.take<uint64_t>(std::max<size_t>(1, size_t(L - 1)) * tstride));  // 7 temp
     v.push_back(A.take<uint32_t>(sz_cnt));                // 8 counts
     v.push_back(A.take<uint64_t>(sz_cnt));                // 9 pref
@@ -1383,7 +1439,7 @@ void build_batch_device(pwmhip_ctx* ctx, const void* const* d_texts, const uint6
   PWM_HIP_TRY(hipMemcpyAsync(b[3], out0.data(), T * 8, hipMemcpyHostToDevice, s));
   PWM_HIP_TRY(hipMemcpyAsync(b[4], stride.data(), T * 8, hipMemcpyHostToDevice, s));
   PWM_HIP_TRY(hipMemcpyAsync(b[5], wave_base.data(), (T + 1) * 8, hipMemcpyHostToDevice, s));
-  PWM_HIP_TRY(hipMemsetAsync(b[6], 0, T * 64 * 8, s));
+  PWM_HIP_TRY(hipMemsetAsync(b[6], 0, T * kMaxGroup * 8, s));
   PWM_HIP_TRY(hipMemsetAsync(b[21], 0, 8, s));
   hipLaunchKernelGGL(k_fill_u32, dim3(1), dim3(64), 0, s, static_cast<unsigned int*>(b[22]), 1u, 0u);
 
@@ -1403,7 +1459,6 @@ void build_batch_device(pwmhip_ctx* ctx, const void* const* d_texts, const uint6
   k1.nslot = nslot;
   k1.sym_or = static_cast<unsigned long long*>(b[21]);
   k1.nchunks = C;
-  k1.aligned16 = aligned ? 1 : 0;
   k1.b_chunk_base = U64(0);
   k1.b_text = U64(1);
   k1.b_n = U64(2);
@@ -1449,9 +1504,9 @@ void build_batch_device(pwmhip_ctx* ctx, const void* const* d_texts, const uint6
   }
   PWM_HIP_TRY(hipGetLastError());
   PWM_HIP_TRY(hipEventRecord(ctx->ev[3], s));
-  std::vector<uint64_t> zeros(T * 64);
+  std::vector<uint64_t> zeros(T * kMaxGroup);
   uint64_t sym_or = 0;
-  PWM_HIP_TRY(hipMemcpyAsync(zeros.data(), b[6], T * 64 * 8, hipMemcpyDeviceToHost, s));
+  PWM_HIP_TRY(hipMemcpyAsync(zeros.data(), b[6], T * kMaxGroup * 8, hipMemcpyDeviceToHost, s));
   PWM_HIP_TRY(hipMemcpyAsync(&sym_or, b[21], 8, hipMemcpyDeviceToHost, s));
   PWM_HIP_TRY(hipStreamSynchronize(s));
   float ms = 0;
@@ -1465,7 +1520,7 @@ void build_batch_device(pwmhip_ctx* ctx, const void* const* d_texts, const uint6
     throw status_error{PWMHIP_ERR_SYMBOL_RANGE, "symbol >= 2^levels"};
   for (uint64_t t = 0; t < T; ++t) {
     outs[t].n_zeros = (tree || !sizes[t]) ? 0 : L;
-    for (uint32_t l = 0; l < L; ++l) outs[t].zeros[l] = (tree || !sizes[t]) ? 0 : zeros[t * 64 + l];
+    for (uint32_t l = 0; l < L; ++l) outs[t].zeros[l] = (tree || !sizes[t]) ? 0 : zeros[t * kMaxGroup + l];
   }
   guard.p = nullptr;
 }

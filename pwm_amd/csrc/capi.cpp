@@ -191,8 +191,17 @@ int pwmhip_build(pwmhip_ctx* ctx, const void* text, uint64_t size, uint32_t word
 int pwmhip_build_batch(pwmhip_ctx* ctx, const void* const* texts, const uint64_t* sizes, uint64_t count,
                        uint32_t word_width, uint32_t levels, uint32_t algo, uint32_t flags, pwmhip_ws* outs) {
   if (!ctx || (count && (!texts || !sizes || !outs))) return PWMHIP_ERR_INVALID;
-  if (flags & PWMHIP_FLAG_REUSE_OUTPUT) return PWMHIP_ERR_INVALID;
-  for (uint64_t t = 0; t < count; ++t) std::memset(&outs[t], 0, sizeof(pwmhip_ws));
+  void* reuse_ptr = nullptr;
+  uint64_t reuse_bytes = 0;
+  if ((flags & PWMHIP_FLAG_REUSE_OUTPUT) && count && outs[0].d_words && outs[0].alloc_bytes) {
+    reuse_ptr = outs[0].d_words;
+    reuse_bytes = outs[0].alloc_bytes;
+  }
+  for (uint64_t t = 0; t < count; ++t) {
+    std::free(outs[t].code_len);
+    std::free(outs[t].code_word);
+    std::memset(&outs[t], 0, sizeof(pwmhip_ws));
+  }
   if (word_width != 1 && word_width != 2 && word_width != 4 && word_width != 8) return PWMHIP_ERR_INVALID;
   if (algo > PWMHIP_ALGO_HUFF_DD_PC) return PWMHIP_ERR_INVALID;
   const bool huff = pwm::is_huffman_algo(algo);
@@ -202,6 +211,10 @@ int pwmhip_build_batch(pwmhip_ctx* ctx, const void* const* texts, const uint64_t
     if (sizes[t] > 0 && !texts[t]) return PWMHIP_ERR_INVALID;
   if (count == 0) return PWMHIP_OK;
   if (huff || levels > uint32_t(pwm::kMaxGroup)) {
+    if (reuse_ptr) {
+      (void)hipSetDevice(ctx->device);
+      (void)hipFree(reuse_ptr);
+    }
     // shapes the batched kernels do not cover: one build per text, each
     // structure owning its memory
     for (uint64_t t = 0; t < count; ++t) {
@@ -234,7 +247,7 @@ int pwmhip_build_batch(pwmhip_ctx* ctx, const void* const* texts, const uint64_t
       }
     }
     pwm::build_batch_device(ctx, d_texts.data(), sizes, count, word_width, levels, tree,
-                            (flags & PWMHIP_FLAG_CHECK_RANGE) != 0, outs);
+                            (flags & PWMHIP_FLAG_CHECK_RANGE) != 0, outs, reuse_ptr, reuse_bytes);
     if (!tree && pwm::zeros_last_is_zero(algo))
       for (uint64_t t = 0; t < count; ++t)
         if (outs[t].levels) outs[t].zeros[levels - 1] = 0;

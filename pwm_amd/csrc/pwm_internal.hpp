@@ -92,9 +92,12 @@ void build_plain_device(pwmhip_ctx* ctx, const void* d_text, uint64_t n, uint32_
 
 // Builds T texts (device pointers) with one launch per phase.  All
 // structures live in one allocation owned by outs[0] (alloc_bytes > 0, the
-// others are views with alloc_bytes == 0).  levels <= kMaxGroup.
+// others are views with alloc_bytes == 0).  levels <= kMaxGroup.  `reuse`
+// (reuse_bytes) is an earlier batch's block, taken over when large enough
+// and freed otherwise.
 void build_batch_device(pwmhip_ctx* ctx, const void* const* d_texts, const uint64_t* sizes, uint64_t T,
-                        uint32_t width, uint32_t L, bool tree, bool check_range, pwmhip_ws* outs);
+                        uint32_t width, uint32_t L, bool tree, bool check_range, pwmhip_ws* outs, void* reuse,
+                        uint64_t reuse_bytes);
 
 // Allocates ws->d_words for `levels` levels of `bits[l]` bits each and fills
 // the offsets.
