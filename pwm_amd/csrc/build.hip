@@ -37,6 +37,7 @@
 
 #include "pwm_internal.hpp"
 
+#include <cstdlib>
 #include <cstring>
 #include <algorithm>
 
@@ -557,6 +558,226 @@ __global__ __launch_bounds__(64 * waves_for<NP>()) void k_local_build(K1Params p
     for (int d = 32; d >= 1; d >>= 1) my_or |= __shfl_xor(my_or, d, 64);
     if (lane == 0 && my_or) atomicOr(p.sym_or, (unsigned long long)my_or);
   }
+}
+
+// ---- K1 for few levels -----------------------------------------------------
+// All G levels of the pass come from the text (no payload planes), G <= 4.
+// Every plane stays in registers.  Sub-level j of a chunk is the chunk's
+// symbols ordered by bucket q = (bit j-1 .. bit 0 of the symbol's top
+// levels), stable inside a bucket -- the order the level-by-level stable
+// partition produces -- so it is assembled directly: for every bucket q the
+// lane's bits of plane j under the bucket mask are compressed and OR-ed into
+// the wave's LDS word array at the bucket's start plus the lane's exclusive
+// prefix.  Bucket sizes and prefixes come from packed 16-bit wave scans.
+
+// pext of a 32-bit word through the byte table (lut at LDS address 0).
+__device__ __forceinline__ uint32_t compress32(uint32_t x, uint32_t m, const uint8_t* lut) {
+  uint32_t r = lut[__builtin_amdgcn_perm(m, x, 0x0c0c0400u)];
+  uint32_t sft = __builtin_popcount(m & 0xffu);
+  r |= uint32_t(lut[__builtin_amdgcn_perm(m, x, 0x0c0c0501u)]) << sft;
+  sft = __builtin_popcount(m & 0xffffu);
+  r |= uint32_t(lut[__builtin_amdgcn_perm(m, x, 0x0c0c0602u)]) << sft;
+  sft = __builtin_popcount(m & 0xffffffu);
+  r |= uint32_t(lut[__builtin_amdgcn_perm(m, x, 0x0c0c0703u)]) << sft;
+  return r;
+}
+
+__device__ __forceinline__ uint64_t compress64h(uint64_t x, uint64_t m, const uint8_t* lut) {
+  const uint32_t lo = compress32(uint32_t(x), uint32_t(m), lut);
+  const uint32_t hi = compress32(uint32_t(x >> 32), uint32_t(m >> 32), lut);
+  return uint64_t(lo) | (uint64_t(hi) << __builtin_popcount(uint32_t(m)));
+}
+
+template <int G>
+__global__ __launch_bounds__(64 * 16) void k_local_small(K1Params p) {
+  constexpr int WAVES = 16;
+  constexpr int NSUB = G > 1 ? G - 1 : 1;  // sub-levels 1..G-1
+  constexpr int NQ = 1 << (G - 1);          // buckets of the last sub-level
+  constexpr int NF = NQ + 1;                // scan fields: bucket counts + last level's zeros
+  constexpr int NS = (NF + 1) / 2;          // packed scans
+  __shared__ uint8_t lut[65536];
+  __shared__ uint64_t sbuf[WAVES][NSUB][64];
+
+  const int lane = threadIdx.x & 63;
+  const int wid = threadIdx.x >> 6;
+  for (int i = threadIdx.x; i < 65536; i += 64 * WAVES)
+    lut[i] = uint8_t(compress8(uint32_t(i) & 255u, uint32_t(i) >> 8));
+  __syncthreads();
+  uint64_t(&buf)[NSUB][64] = sbuf[wid];
+
+  uint32_t zsum[G];
+#pragma unroll
+  for (int j = 0; j < G; ++j) zsum[j] = 0;
+  uint32_t zchunks = 0;
+  uint64_t my_or = 0;
+  const uint64_t gw = uint64_t(blockIdx.x) * WAVES + wid;
+  const uint64_t tw = uint64_t(gridDim.x) * WAVES;
+  uint64_t grab_end = 0;
+  auto next_chunk = [&](uint64_t prev, bool first) -> uint64_t {
+    if (!p.work) return first ? gw : prev + tw;
+    if (!first && prev + 1 < grab_end) return prev + 1;
+    uint32_t q = 0;
+    if (lane == 0) q = atomicAdd(p.work, 1u);
+    q = __builtin_amdgcn_readfirstlane(q);
+    const uint64_t c0 = uint64_t(q) * kGrab;
+    grab_end = min<uint64_t>(c0 + kGrab, p.nchunks);
+    return c0;
+  };
+  uint32_t hint = 0;
+  unsigned long long* zdst = nullptr;
+  auto flush_zeros = [&]() {
+    if (zdst && lane == 0) {
+#pragma unroll
+      for (int j = 0; j < G; ++j)
+        if (zsum[j]) atomicAdd(zdst + j, (unsigned long long)zsum[j]);
+    }
+#pragma unroll
+    for (int j = 0; j < G; ++j) zsum[j] = 0;
+    zchunks = 0;
+  };
+  const bool fast_mode = p.width == 1;
+  auto full_chunk = [&](const ChunkView& v) {
+    return v.base + kChunkSyms <= v.n && (reinterpret_cast<uintptr_t>(v.text) & 15) == 0;
+  };
+  auto load_chunk = [&](const ChunkView& v, uint4* d) {
+    const auto src =
+        gmem(reinterpret_cast<const uint4*>(static_cast<const uint8_t*>(v.text) + v.base + 64ull * lane));
+    d[0] = src[0];
+    d[1] = src[1];
+    d[2] = src[2];
+    d[3] = src[3];
+  };
+  uint4 pf[4];
+  uint64_t c = next_chunk(0, true);
+  ChunkView cv;
+  if (c < p.nchunks) {
+    cv = chunk_view(p, c, hint);
+    if (fast_mode && full_chunk(cv)) load_chunk(cv, pf);
+  }
+
+  while (c < p.nchunks) {
+    wave_sync();  // the previous chunk's LDS reads are done
+    const uint64_t cn = next_chunk(c, false);
+    ChunkView nv;
+    if (cn < p.nchunks) nv = chunk_view(p, cn, hint);
+    if (cv.zout != zdst || zchunks == (1u << 19)) {
+      flush_zeros();
+      zdst = cv.zout;
+    }
+    ++zchunks;
+    const bool have_cur = fast_mode && full_chunk(cv);
+    const uint32_t Ec = uint32_t(min<uint64_t>(kChunkSyms, cv.n - cv.base));
+    const uint32_t nw = (Ec + 63) >> 6;
+    const int cnt = max(0, min(64, int(Ec) - 64 * lane));
+
+    uint64_t P0[G];
+#pragma unroll
+    for (int k = 0; k < G; ++k) P0[k] = 0;
+    if (have_cur) {
+      planes_from_bytes<G>(p, pf, P0, my_or);
+      if (cn < p.nchunks && full_chunk(nv)) load_chunk(nv, pf);  // prefetch
+    } else {
+      if (cnt > 0) {
+        const uint64_t e0 = cv.base + 64ull * lane;
+        switch (p.width) {
+          case 1: load_planes_text<G, uint8_t>(p, cv.text, e0, cnt, P0, my_or); break;
+          case 2: load_planes_text<G, uint16_t>(p, cv.text, e0, cnt, P0, my_or); break;
+          case 4: load_planes_text<G, uint32_t>(p, cv.text, e0, cnt, P0, my_or); break;
+          default: load_planes_text<G, uint64_t>(p, cv.text, e0, cnt, P0, my_or); break;
+        }
+      }
+      if (cn < p.nchunks && fast_mode && full_chunk(nv)) load_chunk(nv, pf);
+    }
+    const uint64_t vm = cnt >= 64 ? ~0ull : (cnt <= 0 ? 0ull : ((1ull << cnt) - 1));
+    if (lane < int(nw)) gmem(cv.lvl0)[lane] = brev64(P0[0]);
+#pragma unroll
+    for (int j = 0; j < NSUB; ++j) buf[j][lane] = 0;
+
+    // bucket masks of the last sub-level, M[q] = prod_i (bit i of q ? P_i : ~P_i)
+    uint64_t M[NQ];
+    M[0] = vm;
+#pragma unroll
+    for (int i = 0; i < G - 1; ++i) {
+#pragma unroll
+      for (int q = (1 << i) - 1; q >= 0; --q) {
+        M[q + (1 << i)] = M[q] & P0[i];
+        M[q] = M[q] & ~P0[i];
+      }
+    }
+    uint32_t f[2 * NS];
+#pragma unroll
+    for (int q = 0; q < NQ; ++q) f[q] = __popcll(M[q]);
+    f[NQ] = __popcll(~P0[G - 1] & vm);
+    if (NF < 2 * NS) f[2 * NS - 1] = 0;
+    uint32_t ex[2 * NS], tot[2 * NS];
+#pragma unroll
+    for (int s2 = 0; s2 < NS; ++s2) {
+      const uint32_t v = f[2 * s2] | (f[2 * s2 + 1] << 16);
+      const uint32_t inc = wave_incl_scan(v);
+      const uint32_t t = __builtin_amdgcn_readlane(inc, 63);
+      const uint32_t e = inc - v;
+      ex[2 * s2] = e & 0xffffu;
+      ex[2 * s2 + 1] = e >> 16;
+      tot[2 * s2] = t & 0xffffu;
+      tot[2 * s2 + 1] = t >> 16;
+    }
+    // zeros of every level: level j < G-1 has bit j == 0 in buckets q < 2^j
+    // of sub-level j+1, i.e. q with bit j clear among the last sub-level's
+    {
+#pragma unroll
+      for (int j = 0; j < G - 1; ++j) {
+        uint32_t z = 0;
+#pragma unroll
+        for (int q = 0; q < NQ; ++q)
+          if (!((q >> j) & 1)) z += tot[q];
+        zsum[j] += z;
+      }
+      zsum[G - 1] += tot[NQ];
+    }
+
+    // sub-level j (1..G-1): buckets q < 2^j; counts/prefixes are sums over
+    // the last sub-level's buckets that extend q
+    uint32_t cslot[NQ > 1 ? 2 * NQ - 2 : 1];
+#pragma unroll
+    for (int j = 1; j < G; ++j) {
+      const int nb = 1 << j;
+      uint32_t start = 0;
+#pragma unroll
+      for (int q = 0; q < nb; ++q) {
+        uint32_t T = 0, E = 0;
+#pragma unroll
+        for (int r = q; r < NQ; r += nb) {
+          T += tot[r];
+          E += ex[r];
+        }
+        uint64_t mq = vm;
+#pragma unroll
+        for (int i = 0; i < j; ++i) mq &= ((q >> i) & 1) ? P0[i] : ~P0[i];
+        const uint32_t len = __popcll(mq);
+        or_bits(buf[j - 1], start + E, compress64h(P0[j], mq, lut), len);
+        cslot[nb - 2 + q] = T;
+        start += T;
+      }
+    }
+    wave_sync();
+    if (lane < int(nw)) {
+#pragma unroll
+      for (int j = 1; j < G; ++j)
+        gmem(p.temp_levels)[(j - 1) * p.temp_stride + c * P + lane] = brev64(buf[j - 1][lane]);
+    }
+    if (G > 1) {
+      uint32_t v = 0;
+#pragma unroll
+      for (int i = 0; i < 2 * NQ - 2; ++i) v = lane == i ? cslot[i] : v;
+      if (lane < 2 * NQ - 2) gmem(p.counts)[c * p.nslot + lane] = v;
+    }
+    c = cn;
+    cv = nv;
+  }
+  flush_zeros();
+#pragma unroll
+  for (int d = 32; d >= 1; d >>= 1) my_or |= __shfl_xor(my_or, d, 64);
+  if (lane == 0 && my_or) atomicOr(p.sym_or, (unsigned long long)my_or);
 }
 
 // ---- K2 --------------------------------------------------------------------
@@ -1107,6 +1328,16 @@ uint64_t scan_scratch_words(uint64_t nchunks, uint32_t nslot) {
 
 namespace {
 
+// PWMHIP_GENERIC_K1=1 routes every pass through the generic local build
+// (for comparisons).
+bool small_k1_disabled() {
+  static const bool off = [] {
+    const char* e = std::getenv("PWMHIP_GENERIC_K1");
+    return e && e[0] == '1';
+  }();
+  return off;
+}
+
 template <int NP>
 void launch_k1(pwmhip_ctx* ctx, const K1Params& prm) {
   constexpr int WAVES = waves_for<NP>();
@@ -1115,8 +1346,23 @@ void launch_k1(pwmhip_ctx* ctx, const K1Params& prm) {
   PWM_HIP_TRY(hipGetLastError());
 }
 
+template <int G>
+void launch_small(pwmhip_ctx* ctx, const K1Params& prm) {
+  const uint64_t grid = std::min<uint64_t>((prm.nchunks + 15) / 16, uint64_t(ctx->num_cus));
+  hipLaunchKernelGGL(k_local_small<G>, dim3(uint32_t(grid)), dim3(64 * 16), 0, ctx->stream, prm);
+  PWM_HIP_TRY(hipGetLastError());
+}
+
 void dispatch_k1(pwmhip_ctx* ctx, const K1Params& prm) {
   const uint32_t np = prm.np;
+  if (!prm.planes_in && prm.g == np && np <= 4 && !small_k1_disabled()) {
+    switch (np) {
+      case 1: launch_small<1>(ctx, prm); return;
+      case 2: launch_small<2>(ctx, prm); return;
+      case 3: launch_small<3>(ctx, prm); return;
+      default: launch_small<4>(ctx, prm); return;
+    }
+  }
   if (np <= 1) launch_k1<1>(ctx, prm);
   else if (np <= 2) launch_k1<2>(ctx, prm);
   else if (np <= 3) launch_k1<3>(ctx, prm);
