@@ -994,7 +994,8 @@ struct GatherParams {
   uint64_t n;
 };
 
-constexpr uint32_t kGatherWindows = 16;  // 64-word windows per wave
+constexpr uint32_t kGatherWindows = 32;  // 64-word windows per wave
+constexpr uint32_t kGroup = 4;           // windows whose loads are issued together
 
 __device__ __forceinline__ uint64_t readlane64(uint64_t v, uint32_t lane) {
   const uint32_t lo = __builtin_amdgcn_readlane(uint32_t(v), lane);
@@ -1002,13 +1003,14 @@ __device__ __forceinline__ uint64_t readlane64(uint64_t v, uint32_t lane) {
   return (uint64_t(hi) << 32) | lo;
 }
 
-// One destination word assembled from the runs cached in the wave (entry i
-// of the cache in lane i: bd = destination start, bs = source start).  All
-// lanes call it (shuffles read 0 from inactive lanes); `live` lanes own a
-// word of the level.
+// Contributions of the cached runs 0..62 to one destination word [ws, we)
+// (lanes hold entry i of the cache in bd/bs; run i ends where run i+1
+// starts).  Every lane calls it (shuffles read 0 from inactive lanes);
+// `pending` lanes own an unfinished word and stay pending when the word
+// continues past the cache.
 __device__ __forceinline__ uint64_t gather_word(const uint64_t* src, uint64_t bd, uint64_t bs, uint64_t ws,
-                                                uint64_t we, bool live) {
-  // last cached entry with start <= ws
+                                                uint64_t we, bool& pending) {
+  // last cached entry with start <= ws (entry 0 if none)
   uint32_t idx = 0;
 #pragma unroll
   for (uint32_t step = 32; step >= 1; step >>= 1) {
@@ -1019,15 +1021,22 @@ __device__ __forceinline__ uint64_t gather_word(const uint64_t* src, uint64_t bd
   uint64_t d1 = __shfl(bd, int(idx + 1 < 64 ? idx + 1 : 63), 64);
   uint64_t s0 = __shfl(bs, int(idx), 64);
   uint64_t v = 0;
-  bool more = live;
+  bool more = pending;
   while (__ballot(more)) {  // wave-uniform loop: shuffles below see every lane
     if (more) {
-      const uint64_t a = max(d0, ws), b = min(d1, we);
-      if (a < b) {
-        const int64_t sb = int64_t(s0) - int64_t(d0) + int64_t(ws);
-        v |= window64(src, sb) & range_mask(uint32_t(a - ws), uint32_t(b - ws));
+      if (idx >= 63) {
+        more = false;  // run 63 ends outside the cache
+      } else {
+        const uint64_t a = max(d0, ws), b = min(d1, we);
+        if (a < b) {
+          const int64_t sb = int64_t(s0) - int64_t(d0) + int64_t(ws);
+          v |= window64(src, sb) & range_mask(uint32_t(a - ws), uint32_t(b - ws));
+        }
+        if (d1 >= we) {
+          more = false;
+          pending = false;
+        }
       }
-      if (d1 >= we || idx + 2 > 64) more = false;
     }
     const uint32_t nidx = more ? idx + 1 : idx;
     const uint64_t nd1 = __shfl(bd, int(nidx + 1 < 64 ? nidx + 1 : 63), 64);
@@ -1042,16 +1051,68 @@ __device__ __forceinline__ uint64_t gather_word(const uint64_t* src, uint64_t bd
   return v;
 }
 
-// Writes the wave's destination words [W0, W0 + 64*kGatherWindows) of `nplanes`
+// 64-entry run cache of a wave: lane i holds entry base+i (bd = destination
+// start, n past the last entry; bs = source start).
+struct RunCache {
+  const uint64_t* tab_dst;
+  const uint64_t* tab_src;
+  uint64_t NE, n, base, bd, bs;
+  __device__ void load(uint32_t lane) {
+    bd = base + lane < NE ? tab_dst[base + lane] : n;
+    bs = base + lane < NE ? tab_src[base + lane] : 0;
+  }
+};
+
+// One window (64 words from w0) whatever its number of runs.
+__device__ __forceinline__ void gather_window(const uint64_t* src, uint64_t* dst, RunCache& rc, uint64_t w0) {
+  const uint32_t lane = threadIdx.x & 63;
+  const uint64_t n = rc.n;
+  const uint64_t sw0 = w0 << 6, sw1 = min<uint64_t>(sw0 + 4096, n);
+  const uint64_t ws = (w0 + lane) << 6, we = min<uint64_t>(ws + 64, n);
+  const bool live = ws < n;
+  // cached runs overlapping [sw0, sw1): ef..el (bd[0] <= sw0 holds)
+  uint32_t ef = __popcll(__ballot(rc.bd <= sw0)) - 1;
+  uint32_t el = __popcll(__ballot(rc.bd < sw1)) - 1;
+  if (el == 63 && ef > 0) {  // move the cache to start at the window's first run
+    rc.base += ef;
+    rc.load(lane);
+    ef = 0;
+    el = __popcll(__ballot(rc.bd < sw1)) - 1;
+  }
+  uint64_t v = 0;
+  if (el < 63 && el - ef < 4) {
+    for (uint32_t e = ef; e <= el; ++e) {
+      const uint64_t d0 = readlane64(rc.bd, e), d1 = readlane64(rc.bd, e + 1), s0 = readlane64(rc.bs, e);
+      const uint64_t a = max(d0, ws), b = min(d1, we);
+      if (live && a < b) {
+        const int64_t sb = int64_t(s0) - int64_t(d0) + int64_t(ws);
+        v |= window64(src, sb) & range_mask(uint32_t(a - ws), uint32_t(b - ws));
+      }
+    }
+  } else {
+    bool pending = live;
+    for (;;) {
+      v |= gather_word(src, rc.bd, rc.bs, ws, we, pending);
+      if (!__ballot(pending)) break;
+      rc.base += 63;  // the words continue past cached run 62
+      rc.load(lane);
+    }
+  }
+  if (live) gmem(dst)[w0 + lane] = v;
+}
+
+// Writes the wave's destination words [W0, W0 + 64*nwin) of `nplanes`
 // planes of one level from the level's run table (tab_dst/tab_src, NE
-// entries, destination order).
-__device__ void gather_range(const uint64_t* tab_dst, const uint64_t* tab_src, uint64_t NE, const uint64_t* src0,
+// entries in destination order; the runs tile [0, n)).  Groups of kGroup
+// windows whose runs all sit in the cache go together: every lane locates
+// the runs of its kGroup words (one per window) with uniform compares, then
+// issues all their loads at once.  Other windows go one at a time.
+__device__ __forceinline__ void gather_range(const uint64_t* tab_dst, const uint64_t* tab_src, uint64_t NE, const uint64_t* src0,
                              uint64_t src_stride, uint64_t* dst0, uint64_t dst_stride, uint32_t nplanes, uint64_t n,
-                             uint64_t W0) {
+                             uint64_t W0, uint32_t nwin) {
   const uint32_t lane = threadIdx.x & 63;
   const uint64_t X = W0 << 6;
   if (X >= n || NE == 0) return;
-  const uint64_t Xw = min<uint64_t>(X + 4096ull * kGatherWindows, n);  // end of this wave's range
   // last entry with dst <= X (64-way ballot search)
   uint64_t lo = 0, hi = NE;
   while (hi - lo > 1) {
@@ -1062,54 +1123,76 @@ __device__ void gather_range(const uint64_t* tab_dst, const uint64_t* tab_src, u
     lo += uint64_t(cnt) * step;
     hi = min(hi, lo + step);
   }
-  const uint64_t bd0 = lo + lane < NE ? tab_dst[lo + lane] : n;
-  const uint64_t bs0 = lo + lane < NE ? tab_src[lo + lane] : 0;
-  // fast path: the 64 cached entries reach past the end of the wave's range
-  const bool cached = __builtin_amdgcn_readlane(uint32_t(bd0 >= Xw), 63) != 0;
-  for (uint32_t k = 0; k < nplanes; ++k) {
-    const uint64_t* src = src0 + k * src_stride;
-    uint64_t* dst = dst0 + k * dst_stride;
-    if (cached) {
-#pragma unroll 4
-      for (uint32_t win = 0; win < kGatherWindows; ++win) {
-        const uint64_t w = W0 + 64ull * win + lane;
-        const uint64_t ws = w << 6;
-        const bool live = ws < n;
-        const uint64_t v = gather_word(src, bd0, bs0, ws, min<uint64_t>(ws + 64, n), live);
-        if (live) dst[w] = v;
-      }
-      continue;
-    }
-    // general path: walk the entries with a sliding 64-entry cache
-    uint64_t base = lo, e = lo, bd = bd0, bs = bs0;
-    for (uint32_t win = 0; win < kGatherWindows; ++win) {
-      const uint64_t w = W0 + 64ull * win + lane;
-      const uint64_t sw0 = (W0 + 64ull * win) << 6;
+  for (uint32_t pl = 0; pl < nplanes; ++pl) {
+    const uint64_t* src = src0 + pl * src_stride;
+    uint64_t* dst = dst0 + pl * dst_stride;
+    RunCache rc{tab_dst, tab_src, NE, n, lo, 0, 0};
+    rc.load(lane);
+    for (uint32_t g0 = 0; g0 < nwin; g0 += kGroup) {
+      const uint64_t sw0 = (W0 + 64ull * g0) << 6;
       if (sw0 >= n) break;
-      const uint64_t sw1 = min<uint64_t>(sw0 + 4096, n);
-      const uint64_t ws = w << 6, we = min<uint64_t>(ws + 64, n);
-      uint64_t v = 0;
-      for (;;) {
-        if (e >= NE) break;
-        if (e + 1 - base >= 64) {  // keep entries e and e+1 in the cache
-          base = e;
-          bd = base + lane < NE ? tab_dst[base + lane] : n;
-          bs = base + lane < NE ? tab_src[base + lane] : 0;
+      bool done = false;
+      if (g0 + kGroup <= nwin) {
+        const uint64_t gend = min<uint64_t>(sw0 + 4096ull * kGroup, n);
+        uint32_t ef = __popcll(__ballot(rc.bd <= sw0)) - 1;
+        uint32_t el = __popcll(__ballot(rc.bd < gend)) - 1;
+        if (el == 63 && ef > 0) {
+          rc.base += ef;
+          rc.load(lane);
+          ef = 0;
+          el = __popcll(__ballot(rc.bd < gend)) - 1;
         }
-        const uint64_t d0 = readlane64(bd, uint32_t(e - base));
-        const uint64_t d1 = readlane64(bd, uint32_t(e + 1 - base));  // n past the last entry
-        if (d0 >= sw1) break;
-        if (d1 > sw0) {
-          const uint64_t a = max(d0, ws), b = min(d1, we);
-          if (a < b) {
-            const int64_t sb = int64_t(readlane64(bs, uint32_t(e - base))) - int64_t(d0) + int64_t(ws);
-            v |= window64(src, sb) & range_mask(uint32_t(a - ws), uint32_t(b - ws));
+        if (el < 63) {
+          uint64_t v[kGroup], d1k[kGroup];
+          uint32_t idx[kGroup];
+          const uint64_t ws0 = (W0 + 64ull * g0 + lane) << 6;  // word k starts at ws0 + 4096 k
+#pragma unroll
+          for (int k = 0; k < int(kGroup); ++k) idx[k] = ef;
+          for (uint32_t e = ef + 1; e <= el; ++e) {
+            const uint64_t bde = readlane64(rc.bd, e);
+#pragma unroll
+            for (int k = 0; k < int(kGroup); ++k) idx[k] += bde <= ws0 + 4096ull * k ? 1u : 0u;
           }
+#pragma unroll
+          for (int k = 0; k < int(kGroup); ++k) {
+            const uint64_t ws = ws0 + 4096ull * k;
+            const int64_t dl =
+                int64_t(__shfl(rc.bs, int(idx[k]), 64)) - int64_t(__shfl(rc.bd, int(idx[k]), 64));
+            d1k[k] = __shfl(rc.bd, int(idx[k] + 1), 64);
+            v[k] = ws < n ? window64(src, dl + int64_t(ws)) : 0;
+          }
+#pragma unroll
+          for (int k = 0; k < int(kGroup); ++k) {
+            const uint64_t ws = ws0 + 4096ull * k, we = min<uint64_t>(ws + 64, n);
+            bool more = ws < n && d1k[k] < we;
+            if (ws < n) v[k] &= range_mask(0, uint32_t(min(d1k[k], we) - ws));
+            // the word continues into later runs (run boundaries inside it)
+            uint32_t e = idx[k] + 1;
+            uint64_t d0 = d1k[k];
+            while (__ballot(more)) {
+              const uint64_t d1 = __shfl(rc.bd, int(min(e + 1, 63u)), 64);
+              const uint64_t s0 = __shfl(rc.bs, int(min(e, 63u)), 64);
+              if (more) {
+                const int64_t sb = int64_t(s0) - int64_t(d0) + int64_t(ws);
+                v[k] |= window64(src, sb) & range_mask(uint32_t(d0 - ws), uint32_t(min(d1, we) - ws));
+                more = d1 < we;
+                d0 = d1;
+                ++e;
+              }
+            }
+            if (ws < n) gmem(dst)[ws >> 6] = v[k];
+          }
+          done = true;
         }
-        if (d1 > sw1) break;  // the entry continues into the next window
-        ++e;
       }
-      if (ws < n) dst[w] = v;
+      if (!done) {
+#pragma unroll 1
+        for (uint32_t win = g0; win < min(g0 + kGroup, nwin); ++win) {
+          const uint64_t w0 = W0 + 64ull * win;
+          if ((w0 << 6) >= n) break;
+          gather_window(src, dst, rc, w0);
+        }
+      }
     }
   }
 }
@@ -1117,7 +1200,7 @@ __device__ void gather_range(const uint64_t* tab_dst, const uint64_t* tab_src, u
 __global__ __launch_bounds__(256) void k_gather(GatherParams p) {
   const uint64_t W0 = (uint64_t(blockIdx.x) * 4 + (threadIdx.x >> 6)) * 64 * kGatherWindows;
   gather_range(p.tab_dst, p.tab_src, p.level_ne[p.j], p.src, p.src_stride, p.dst, p.dst_stride, p.nplanes, p.n,
-               W0);
+               W0, kGatherWindows);
 }
 
 // Batch: waves are assigned to texts by wave_base (per text), level j.
@@ -1150,7 +1233,8 @@ __global__ __launch_bounds__(256) void k_gather_batch(BatchGatherParams p) {
   const uint64_t W0 = (wv - p.wave_base[t]) * 64 * kGatherWindows;
   const uint64_t toff = uint64_t((1u << p.j) - 2) * p.nchunks + p.chunk_base[t] * (1ull << p.j);
   gather_range(p.tab_dst + toff, p.tab_src + toff, p.level_ne[uint64_t(t) * 8 + p.j],
-               p.src + (p.j - 1) * p.src_stride, 0, p.out + p.out0[t] + p.j * p.stride[t], 0, 1, p.text_n[t], W0);
+               p.src + (p.j - 1) * p.src_stride, 0, p.out + p.out0[t] + p.j * p.stride[t], 0, 1, p.text_n[t], W0,
+               kGatherWindows);
 }
 
 // Fused offsets for the run table: per (tile of 1024 chunks, slot) the sums of
