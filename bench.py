@@ -8,9 +8,11 @@ construction of all levels from the text resident in HBM.
     python bench.py [--gpus 1] [--steps K] [--warmup W] [--config NAME]
 
 Prints one JSON line.  `value` is input GB/s (n * bytes per symbol / time per
-step); `roofline` relates the minimal algorithmic HBM bytes of a build
-(text read once + every level written once) to the device time of the
-build's kernels; `cpu_baseline` is the reference's own OpenMP algorithms
+step); `roofline` is that of the dominant kernel, the local build (it reads
+the text once and writes every level's bits once: n*width + levels*n/8
+bytes), over its own device time (HIP events on the build stream);
+`config.build_roofline` relates the same algorithmic bytes to the device
+time of the whole build; `cpu_baseline` is the reference's own OpenMP algorithms
 (compiled from its headers, oracle/_ref) on a bounded prefix of the same
 text, on this host's cores.
 """
@@ -73,11 +75,24 @@ def cpu_baseline(text, levels, tree, width, sample_bytes):
 
 
 def traffic_from_profiles(config):
+    """HBM bytes per local-build launch from the committed counter summary."""
     p = os.path.join(ROOT, "profiles", "%s_traffic.json" % config)
     if os.path.exists(p):
         with open(p) as f:
-            return json.load(f).get("bytes_per_build")
+            return json.load(f).get("local_build_hbm_bytes")
     return None
+
+
+def rooflines(b_min, phases_ms, total_ms, config):
+    """Dominant kernel (local build, phase 0) and whole-build figures."""
+    k1 = b_min / (phases_ms[0] / 1e3) / 1e9
+    build = b_min / (total_ms / 1e3) / 1e9
+    roof = {"bound": "hbm", "kernel": "local build (k_local_small / k_local_build)", "achieved": round(k1, 1),
+            "peak": HBM_PEAK_GBS, "unit": "GB/s", "frac": round(k1 / HBM_PEAK_GBS, 4),
+            "traffic": traffic_from_profiles(config), "algorithmic_bytes": int(b_min),
+            "kernel_ms": round(phases_ms[0], 4)}
+    whole = {"achieved": round(build, 1), "frac": round(build / HBM_PEAK_GBS, 4), "device_ms": round(total_ms, 4)}
+    return roof, whole
 
 
 def batch_sizes(seed, T, median, sigma, lo, hi):
@@ -168,16 +183,24 @@ def run_batch(args):
             devsum += arr[3]
         return devsum
 
-    per_text()  # allocates every output once
-    torch.cuda.synchronize()
-    t0 = time.perf_counter()
-    single_dev_ms = per_text()
-    torch.cuda.synchronize()
-    single_ms = (time.perf_counter() - t0) * 1e3
-
-    # bit-exact: the batched structures equal the per-text ones
     rng = np.random.default_rng(1)
     check = sorted(set(rng.integers(0, T, 200).tolist()) | {int(np.argmax(sizes)), int(np.argmin(sizes))})
+    single_ms = single_dev_ms = None
+    if not args.no_per_text:
+        per_text()  # allocates every output once
+        torch.cuda.synchronize()
+        t0 = time.perf_counter()
+        single_dev_ms = per_text()
+        torch.cuda.synchronize()
+        single_ms = (time.perf_counter() - t0) * 1e3
+    else:
+        # only the checked texts, for the comparison below
+        for i in check:
+            st = lib.pwmhip_build(ctx.h, ptrs[i], csz[i], 1, levels, code, sflags, ctypes.byref(singles[i]))
+            if st != pwm_amd.OK:
+                raise pwm_amd.PwmError(st, "build")
+
+    # bit-exact: the batched structures equal the per-text ones
     for i in check:
         a, b = outs[i], singles[i]
         assert a.levels == b.levels and list(a.zeros[:levels]) == list(b.zeros[:levels]), i
@@ -191,7 +214,7 @@ def run_batch(args):
 
     words = ((sizes + 63) // 64).sum()
     b_min = total + levels * 8 * int(words)
-    achieved = b_min / (dev_ms / 1e3) / 1e9
+    roof, whole = rooflines(b_min, phases, dev_ms, args.config)
     out = {
         "metric": "WM/WT build input GB/s + % HBM peak; batched vs per-text launches, bit-exact",
         "value": round(total / (batch_ms / 1e3) / 1e9, 2),
@@ -213,14 +236,13 @@ def run_batch(args):
                    "algorithm": ("wt_" if tree else "wm_") + algo,
                    "device_ms_per_build": round(dev_ms, 4),
                    "device_ms_phases": {"local_build": phases[0], "offsets": phases[1], "stitch": phases[2]},
-                   "per_text_ms": round(single_ms, 2),
-                   "per_text_device_ms": round(single_dev_ms, 2),
-                   "speedup_vs_per_text": round(single_ms / batch_ms, 2),
+                   "per_text_ms": round(single_ms, 2) if single_ms else None,
+                   "per_text_device_ms": round(single_dev_ms, 2) if single_dev_ms else None,
+                   "speedup_vs_per_text": round(single_ms / batch_ms, 2) if single_ms else None,
                    "checked_texts": len(check),
+                   "build_roofline": whole,
                    "generate_s": round(gen_s, 2)},
-        "roofline": {"bound": "hbm", "achieved": round(achieved, 1), "peak": HBM_PEAK_GBS, "unit": "GB/s",
-                     "frac": round(achieved / HBM_PEAK_GBS, 4),
-                     "traffic": traffic_from_profiles(args.config)},
+        "roofline": roof,
     }
     if not args.no_cpu:
         out["cpu_baseline"] = cpu_baseline_batch(host, offs, sizes, levels, tree, args.cpu_sample_mib << 20)
@@ -232,6 +254,25 @@ def run_batch(args):
     print(json.dumps(out), flush=True)
 
 
+def run_copy(args):
+    """A plain copy of known size: its counter readings per byte scale the
+    construction kernels' readings."""
+    import torch
+    nbytes = args.copy_mib << 20
+    src = torch.ones(nbytes // 8, dtype=torch.int64, device="cuda:0")
+    dst = torch.empty_like(src)
+    for _ in range(args.warmup):
+        dst.copy_(src)
+    torch.cuda.synchronize()
+    t0 = time.perf_counter()
+    for _ in range(args.steps):
+        dst.copy_(src)
+    torch.cuda.synchronize()
+    dt = (time.perf_counter() - t0) / args.steps
+    print(json.dumps({"copy_bytes": nbytes, "ms": round(dt * 1e3, 4),
+                      "GBps_read_plus_write": round(2 * nbytes / dt / 1e9, 1)}), flush=True)
+
+
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
@@ -240,13 +281,21 @@ def main():
     ap.add_argument("--config", default="dna4g_wm", choices=sorted(CONFIGS) + sorted(BATCH_CONFIGS))
     ap.add_argument("--n", type=int, default=0, help="override the text length")
     ap.add_argument("--no-cpu", action="store_true")
+    ap.add_argument("--no-per-text", action="store_true",
+                    help="many-texts configs: skip the one-build-per-text comparison")
     ap.add_argument("--cpu-sample-mib", type=int, default=256)
+    ap.add_argument("--copy-mib", type=int, default=0,
+                    help="only time a plain device-to-device copy of this many MiB (calibrates the "
+                         "HBM byte counters of a profiler run)")
     args = ap.parse_args()
     if args.gpus != 1:
         print("bench.py: this build measures one GPU only; --gpus %d is not supported "
               "(the construction does not shard across GPUs here)" % args.gpus, file=sys.stderr)
         sys.exit(2)
 
+    if args.copy_mib:
+        run_copy(args)
+        return
     if args.config in BATCH_CONFIGS:
         run_batch(args)
         return
@@ -289,7 +338,7 @@ def main():
             assert int(z[0]) == exp0, "zeros[0] mismatch"
 
     b_min = n * width + levels * 8 * ((n + 63) // 64)
-    achieved = b_min / (dev_total / 1e3) / 1e9
+    roof, whole = rooflines(b_min, phases, dev_total, args.config)
     value = n * width / (ms_per_step / 1e3) / 1e9
     out = {
         "metric": "WM/WT build input GB/s + % HBM peak; batched vs per-text launches, bit-exact",
@@ -310,10 +359,9 @@ def main():
             "device_ms_per_build": round(dev_total, 4),
             "device_ms_phases": {"local_build": phases[0], "offsets": phases[1], "stitch": phases[2]},
             "symbols_per_s": round(n / (ms_per_step / 1e3), 1),
+            "build_roofline": whole,
             "generate_s": round(gen_s, 2)},
-        "roofline": {"bound": "hbm", "achieved": round(achieved, 1), "peak": HBM_PEAK_GBS, "unit": "GB/s",
-                     "frac": round(achieved / HBM_PEAK_GBS, 4),
-                     "traffic": traffic_from_profiles(args.config)},
+        "roofline": roof,
     }
     if not args.no_cpu:
         out["cpu_baseline"] = cpu_baseline(text, levels, tree, width, args.cpu_sample_mib << 20)

@@ -185,6 +185,7 @@ struct K1Params {
   unsigned long long* b_zeros;
   uint32_t ntexts;
   unsigned int* work;        // dynamic chunk queue (atomic counter), or nullptr
+  const uint8_t* lut;        // 64 KiB compress table in device memory
 };
 
 // Chunks a wave takes from the work queue at a time.
@@ -348,6 +349,20 @@ __device__ __forceinline__ void wave_exscan2(uint32_t a, uint32_t b, uint32_t& e
   eb = ib - b;
 }
 
+// Copies the device compress table into LDS (16-byte loads), then a barrier.
+template <int WAVES>
+__device__ __forceinline__ void load_lut(uint8_t* lut, const uint8_t* src) {
+  const auto g = gmem(reinterpret_cast<const uint4*>(src));
+  uint4* l = reinterpret_cast<uint4*>(lut);
+  for (int i = threadIdx.x; i < 65536 / 16; i += 64 * WAVES) l[i] = g[i];
+  __syncthreads();
+}
+
+__global__ void k_make_lut(uint8_t* lut) {
+  const uint32_t i = blockIdx.x * blockDim.x + threadIdx.x;
+  if (i < 65536) lut[i] = uint8_t(compress8(i & 255u, i >> 8));
+}
+
 template <int NP>
 struct WaveLds {
   static constexpr int kB = (1 << (NP < kMaxGroup ? NP : kMaxGroup)) + 1;
@@ -367,14 +382,12 @@ constexpr int waves_for() {
 template <int NP>
 __global__ __launch_bounds__(64 * waves_for<NP>()) void k_local_build(K1Params p) {
   constexpr int WAVES = waves_for<NP>();
-  __shared__ uint8_t lut[65536];
+  __shared__ __align__(16) uint8_t lut[65536];
   __shared__ WaveLds<NP> wl[WAVES];
 
   const int lane = threadIdx.x & 63;
   const int wid = threadIdx.x >> 6;
-  for (int i = threadIdx.x; i < 65536; i += 64 * WAVES)
-    lut[i] = uint8_t(compress8(uint32_t(i) & 255u, uint32_t(i) >> 8));
-  __syncthreads();
+  load_lut<WAVES>(lut, p.lut);
   WaveLds<NP>& L = wl[wid];
   uint64_t my_or = 0;
   // zeros per sub-level over this wave's recent chunks (flushed before
@@ -595,14 +608,12 @@ __global__ __launch_bounds__(64 * 16) void k_local_small(K1Params p) {
   constexpr int NQ = 1 << (G - 1);          // buckets of the last sub-level
   constexpr int NF = NQ + 1;                // scan fields: bucket counts + last level's zeros
   constexpr int NS = (NF + 1) / 2;          // packed scans
-  __shared__ uint8_t lut[65536];
+  __shared__ __align__(16) uint8_t lut[65536];
   __shared__ uint64_t sbuf[WAVES][NSUB][64];
 
   const int lane = threadIdx.x & 63;
   const int wid = threadIdx.x >> 6;
-  for (int i = threadIdx.x; i < 65536; i += 64 * WAVES)
-    lut[i] = uint8_t(compress8(uint32_t(i) & 255u, uint32_t(i) >> 8));
-  __syncthreads();
+  load_lut<WAVES>(lut, p.lut);
   uint64_t(&buf)[NSUB][64] = sbuf[wid];
 
   uint32_t zsum[G];
@@ -1486,6 +1497,17 @@ void alloc_structure(pwmhip_ws* ws, uint64_t levels, const uint64_t* bits, uint6
   ws->d_words = static_cast<uint64_t*>(p);
 }
 
+const uint8_t* compress_table(pwmhip_ctx* ctx) {
+  if (!ctx->d_lut) {
+    void* p = nullptr;
+    PWM_HIP_TRY(hipMalloc(&p, 65536));
+    hipLaunchKernelGGL(k_make_lut, dim3(256), dim3(256), 0, ctx->stream, static_cast<uint8_t*>(p));
+    PWM_HIP_TRY(hipGetLastError());
+    ctx->d_lut = static_cast<uint8_t*>(p);
+  }
+  return ctx->d_lut;
+}
+
 void build_plain_device(pwmhip_ctx* ctx, const void* d_text, uint64_t n, uint32_t width,
                         uint32_t L, bool tree, bool check_range, pwmhip_ws* ws) {
   hipStream_t s = ctx->stream;
@@ -1581,6 +1603,7 @@ void build_plain_device(pwmhip_ctx* ctx, const void* d_text, uint64_t n, uint32_
     k1.zeros_out = reinterpret_cast<unsigned long long*>(zeros_dev + l0);
     k1.sym_or = sym_or;
     k1.nchunks = nchunks;
+    k1.lut = compress_table(ctx);
     dispatch_k1(ctx, k1);
     if (gi == 0) PWM_HIP_TRY(hipEventRecord(ctx->ev[1], s));
 
@@ -1789,6 +1812,7 @@ void build_batch_device(pwmhip_ctx* ctx, const void* const* d_texts, const uint6
   k1.nslot = nslot;
   k1.sym_or = static_cast<unsigned long long*>(b[21]);
   k1.nchunks = C;
+  k1.lut = compress_table(ctx);
   k1.b_chunk_base = U64(0);
   k1.b_text = U64(1);
   k1.b_n = U64(2);

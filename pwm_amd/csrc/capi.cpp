@@ -123,6 +123,7 @@ int pwmhip_destroy(pwmhip_ctx* ctx) {
   ctx->text_scratch.release();
   ctx->rs_scratch.release();
   if (ctx->pinned) (void)hipHostFree(ctx->pinned);
+  if (ctx->d_lut) (void)hipFree(ctx->d_lut);
   for (auto& e : ctx->ev)
     if (e) (void)hipEventDestroy(e);
   if (ctx->stream) (void)hipStreamDestroy(ctx->stream);

@@ -79,9 +79,14 @@ struct pwmhip_ctx {
   // pinned staging for small device->host results
   uint64_t* pinned = nullptr;
   size_t pinned_words = 0;
+  // 64 KiB byte-compress table (lut[(m << 8) | x] = pext(x, m)), filled once
+  uint8_t* d_lut = nullptr;
 };
 
 namespace pwm {
+
+// The context's compress table, computed on first use.
+const uint8_t* compress_table(pwmhip_ctx* ctx);
 
 // ---- plain (non-Huffman) construction -------------------------------------
 // Builds `levels` levels of a wavelet matrix (tree=false) or tree (tree=true)
