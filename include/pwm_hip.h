@@ -42,9 +42,12 @@ extern "C" {
  *     compute it: construction/ppc.hpp:19-90, pc_ss.hpp:13-33,
  *     ppc_ss.hpp:19-70) and the true count for the others;
  *   - the HUFF_* algorithms build Huffman-shaped structures and ignore
- *     `levels` (wx_huff_pc.hpp:45-80).  HUFF_PC / HUFF_PC_SS / HUFF_DD_PC
- *     reproduce huff_pc's border bookkeeping (zeros included); HUFF_NAIVE
- *     gives huff_naive's true zero counts. */
+ *     `levels` (wx_huff_pc.hpp:45-80).  HUFF_PC, HUFF_PS and HUFF_DD_PC
+ *     reproduce huff_pc's border bookkeeping (zeros included; the
+ *     reference's huff_dd_pc matches it at one OpenMP thread, its output
+ *     depends on the thread count above that); HUFF_NAIVE gives
+ *     huff_naive's levels and true zero counts; HUFF_PC_SS gives
+ *     huff_naive's levels with zeros[levels-1] = 0 (huff_pc_ss.hpp:18-121). */
 enum pwmhip_algo {
   PWMHIP_ALGO_PC = 0,
   PWMHIP_ALGO_PS = 1,

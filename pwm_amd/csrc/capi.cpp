@@ -166,8 +166,14 @@ int pwmhip_build(pwmhip_ctx* ctx, const void* text, uint64_t size, uint32_t word
       reuse_ptr = nullptr;
     }
     if (huff) {
-      const bool quirks = algo != PWMHIP_ALGO_HUFF_NAIVE;
+      // huff_pc's border bookkeeping (huff_pc.hpp:19-83) is shared by
+      // huff_ps and huff_dd_pc (the latter at one thread: with more its
+      // output depends on the thread count); huff_pc_ss (huff_pc_ss.hpp:
+      // 18-121) gives huff_naive's levels with zeros[levels-1] left at 0.
+      const bool quirks = algo == PWMHIP_ALGO_HUFF_PC || algo == PWMHIP_ALGO_HUFF_PS ||
+                          algo == PWMHIP_ALGO_HUFF_DD_PC;
       pwm::build_huffman_device(ctx, d_text, size, word_width, tree, quirks, out);
+      if (algo == PWMHIP_ALGO_HUFF_PC_SS && out->n_zeros) out->zeros[out->n_zeros - 1] = 0;
       return;
     }
     if (size == 0) {  // wx_ppc.hpp:39-45: empty structure
