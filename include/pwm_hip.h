@@ -35,6 +35,7 @@ extern "C" {
 #define PWMHIP_ERR_SYMBOL_RANGE -5 /* a symbol >= 2^levels                  */
 #define PWMHIP_ERR_UNSUPPORTED -6  /* input the reference leaves undefined  */
 #define PWMHIP_ERR_BUDGET -7       /* request exceeds the device budget     */
+#define PWMHIP_ERR_TIMEOUT -8      /* wait ended before the request did     */
 
 /* Construction algorithms.  They produce identical level bits; they differ
  * only where the reference's outputs differ:
@@ -180,6 +181,62 @@ int pwmhip_reduce_alphabet(pwmhip_ctx* ctx, void* d_text, uint64_t size, uint32_
 int pwmhip_build_batch(pwmhip_ctx* ctx, const void* const* texts, const uint64_t* sizes,
                        uint64_t count, uint32_t word_width, uint32_t levels, uint32_t algo,
                        uint32_t flags, pwmhip_ws* outs);
+
+/* ---- serving ---------------------------------------------------------------
+ * No counterpart in the reference.  A request loop on one device: builds and
+ * query batches are submitted from any thread and return at once; a
+ * scheduler admits builds while the HBM budget allows (resident indices +
+ * outputs of builds in flight + each worker's scratch allowance), groups
+ * waiting builds of one shape (width, levels, algorithm, tree) into batched
+ * launches, runs them on `workers` streams concurrently, keeps every built
+ * index (levels + rank/select directories) resident until released, and
+ * answers waiting queries of all ready indices together in one launch.
+ * Each result equals what the same request gives alone (pwmhip_build +
+ * pwmhip_query).  Plain algorithms only (Huffman shapes have no queries).
+ * Host buffers passed in (texts, query arrays, result arrays) must stay
+ * valid until the request completes. */
+typedef struct pwmhip_server pwmhip_server;
+
+typedef struct pwmhip_server_config {
+  int device;
+  uint64_t hbm_budget_bytes;    /* 0: 200 GB                                    */
+  uint32_t workers;             /* concurrent streams; 0: 2                     */
+  uint32_t max_batch_texts;     /* texts per batched launch; 0: 1024            */
+  uint64_t max_batch_bytes;     /* text bytes per batched launch; 0: 512 MiB    */
+  uint64_t max_query_batch;     /* queries per launch; 0: 8 Mi                  */
+} pwmhip_server_config;
+
+typedef struct pwmhip_server_stats {
+  uint64_t builds_done, builds_failed, queries_done, query_requests_done;
+  uint64_t build_launches, query_launches;
+  double build_p50_ms, build_p99_ms;   /* submit -> index ready            */
+  double query_p50_ms, query_p99_ms;   /* submit -> results written         */
+  uint64_t bytes_in_use, bytes_peak;   /* against the budget                */
+  double mean_texts_per_launch;
+} pwmhip_server_stats;
+
+int pwmhip_server_create(const pwmhip_server_config* cfg, pwmhip_server** out);
+/* Finishes all submitted work, then frees everything. */
+int pwmhip_server_destroy(pwmhip_server* s);
+/* Submits a build (arguments as pwmhip_build; host text).  *index_id names
+ * the resident index.  PWMHIP_ERR_BUDGET if the request can never fit. */
+int pwmhip_server_build(pwmhip_server* s, const void* text, uint64_t size, uint32_t word_width,
+                        uint32_t levels, uint32_t algo, uint32_t flags, uint64_t* index_id);
+/* Submits n symbol queries (as pwmhip_query) against an index, which may
+ * still be building; results land in `result` (host). */
+int pwmhip_server_query(pwmhip_server* s, uint64_t index_id, const uint32_t* op, const uint64_t* a,
+                        const uint64_t* b, uint64_t n, uint64_t* result, uint64_t* request_id);
+/* Waits until request `id` (an index id or a query request id) completes;
+ * returns its status.  timeout_ms < 0 waits without limit; on timeout
+ * returns PWMHIP_ERR_TIMEOUT. */
+int pwmhip_server_wait(pwmhip_server* s, uint64_t id, int64_t timeout_ms);
+/* Copies level `level` of a ready index to host memory. */
+int pwmhip_server_download(pwmhip_server* s, uint64_t index_id, uint64_t level, uint64_t* host_words);
+/* Metadata of a ready index (device pointers stay owned by the server). */
+int pwmhip_server_index_info(pwmhip_server* s, uint64_t index_id, pwmhip_ws* info);
+/* Drops a resident index (after its queries complete). */
+int pwmhip_server_release(pwmhip_server* s, uint64_t index_id);
+int pwmhip_server_stats_get(pwmhip_server* s, pwmhip_server_stats* out);
 
 /* ---- timing -------------------------------------------------------------
  * Device time of the last build's kernels (HIP events on the context's

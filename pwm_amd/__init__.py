@@ -32,6 +32,7 @@ ERR_HIP = -4
 ERR_SYMBOL_RANGE = -5
 ERR_UNSUPPORTED = -6
 ERR_BUDGET = -7
+ERR_TIMEOUT = -8
 
 FLAG_TREE = 1
 FLAG_TEXT_ON_DEVICE = 2
@@ -72,6 +73,20 @@ class _WS(ctypes.Structure):
     ]
 
 
+class _ServerConfig(ctypes.Structure):
+    _fields_ = [("device", ctypes.c_int), ("hbm_budget_bytes", ctypes.c_uint64), ("workers", ctypes.c_uint32),
+                ("max_batch_texts", ctypes.c_uint32), ("max_batch_bytes", ctypes.c_uint64),
+                ("max_query_batch", ctypes.c_uint64)]
+
+
+class _ServerStats(ctypes.Structure):
+    _fields_ = [(k, ctypes.c_uint64) for k in ("builds_done", "builds_failed", "queries_done",
+                                               "query_requests_done", "build_launches", "query_launches")] + \
+               [(k, ctypes.c_double) for k in ("build_p50_ms", "build_p99_ms", "query_p50_ms", "query_p99_ms")] + \
+               [("bytes_in_use", ctypes.c_uint64), ("bytes_peak", ctypes.c_uint64),
+                ("mean_texts_per_launch", ctypes.c_double)]
+
+
 def _load():
     if not os.path.exists(LIB_PATH):
         raise ImportError("pwm_amd: %s is missing -- run __graft_entry__.build()" % LIB_PATH)
@@ -93,6 +108,15 @@ def _load():
         "pwmhip_build_batch": [vp, vp, vp, u64, u32, u32, u32, u32, ctypes.POINTER(_WS)],
         "pwmhip_last_timing": [vp, ctypes.POINTER(ctypes.c_double)],
         "pwmhip_generate": [u32, u64, u64, u32, u32, vp],
+        "pwmhip_server_create": [ctypes.POINTER(_ServerConfig), ctypes.POINTER(vp)],
+        "pwmhip_server_destroy": [vp],
+        "pwmhip_server_build": [vp, vp, u64, u32, u32, u32, u32, ctypes.POINTER(u64)],
+        "pwmhip_server_query": [vp, u64, vp, vp, vp, u64, vp, ctypes.POINTER(u64)],
+        "pwmhip_server_wait": [vp, u64, ctypes.c_int64],
+        "pwmhip_server_download": [vp, u64, u64, vp],
+        "pwmhip_server_index_info": [vp, u64, ctypes.POINTER(_WS)],
+        "pwmhip_server_release": [vp, u64],
+        "pwmhip_server_stats_get": [vp, ctypes.POINTER(_ServerStats)],
     }
     for name, args in sig.items():
         f = getattr(lib, name)
@@ -460,3 +484,99 @@ class QuerySupport:
 
     def select(self, symbol, occ):
         return int(self.query([2], [symbol], [occ])[0])
+
+
+class Server:
+    """Serving loop on one device (pwmhip_server_*): builds and query batches
+    are submitted and return at once; the native scheduler batches builds of
+    one shape under the HBM budget, keeps indices resident, and answers the
+    queries of all ready indices together.  Inputs stay referenced here until
+    their request completes."""
+
+    def __init__(self, device=0, hbm_budget_bytes=0, workers=0, max_batch_texts=0, max_batch_bytes=0,
+                 max_query_batch=0):
+        cfg = _ServerConfig(device, hbm_budget_bytes, workers, max_batch_texts, max_batch_bytes, max_query_batch)
+        h = ctypes.c_void_p()
+        _check(_lib.pwmhip_server_create(ctypes.byref(cfg), ctypes.byref(h)), "server_create")
+        self.h = h
+        self._keep = {}
+        self._results = {}
+
+    def close(self):
+        if self.h:
+            _lib.pwmhip_server_destroy(self.h)
+            self.h = None
+            self._keep.clear()
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+    def build(self, text, levels, algo="ppc", is_tree=False, check_range=False):
+        """Submits a build of a host text; returns the index id."""
+        text = np.ascontiguousarray(text)
+        if text.dtype not in (np.uint8, np.uint16, np.uint32, np.uint64):
+            raise PwmError(ERR_INVALID, "text must hold unsigned 8/16/32/64-bit symbols")
+        flags = (FLAG_TREE if is_tree else 0) | (FLAG_CHECK_RANGE if check_range else 0)
+        rid = ctypes.c_uint64()
+        _check(_lib.pwmhip_server_build(self.h, text.ctypes.data_as(ctypes.c_void_p), text.size, text.itemsize,
+                                        levels, ALGOS[algo], flags, ctypes.byref(rid)), "server_build")
+        self._keep[rid.value] = (text,)
+        return rid.value
+
+    def query(self, index, op, a, b=None):
+        """Submits symbol queries (op 0 access(a), 1 rank(a, b), 2 select(a, b));
+        returns the request id; results via result(id) after wait(id)."""
+        op = np.ascontiguousarray(op, np.uint32)
+        a = np.ascontiguousarray(a, np.uint64)
+        b = np.ascontiguousarray(b if b is not None else np.zeros(op.size), np.uint64)
+        res = np.zeros(max(1, op.size), np.uint64)
+        rid = ctypes.c_uint64()
+        vp = ctypes.c_void_p
+        _check(_lib.pwmhip_server_query(self.h, index, op.ctypes.data_as(vp), a.ctypes.data_as(vp),
+                                        b.ctypes.data_as(vp), op.size, res.ctypes.data_as(vp),
+                                        ctypes.byref(rid)), "server_query")
+        self._keep[rid.value] = (op, a, b)
+        self._results[rid.value] = res[:op.size]
+        return rid.value
+
+    def wait(self, rid, timeout_ms=-1):
+        st = _lib.pwmhip_server_wait(self.h, rid, timeout_ms)
+        if st != ERR_TIMEOUT:
+            self._keep.pop(rid, None)
+        _check(st, "server_wait")
+
+    def result(self, rid):
+        return self._results.pop(rid)
+
+    def info(self, index):
+        ws = _WS()
+        _check(_lib.pwmhip_server_index_info(self.h, index, ctypes.byref(ws)), "server_index_info")
+        return ws
+
+    def levels(self, index):
+        """Host copies of every level of a ready index."""
+        ws = self.info(index)
+        out = []
+        for l in range(ws.levels):
+            nw = (ws.level_bits[l] + 63) // 64
+            a = np.zeros(max(1, nw), np.uint64)
+            if nw:
+                _check(_lib.pwmhip_server_download(self.h, index, l, a.ctypes.data_as(ctypes.c_void_p)),
+                       "server_download")
+            out.append(a[:nw])
+        return out
+
+    def zeros(self, index):
+        ws = self.info(index)
+        return np.array([ws.zeros[i] for i in range(ws.n_zeros)], np.uint64)
+
+    def release(self, index):
+        _check(_lib.pwmhip_server_release(self.h, index), "server_release")
+
+    def stats(self):
+        st = _ServerStats()
+        _check(_lib.pwmhip_server_stats_get(self.h, ctypes.byref(st)), "server_stats")
+        return {k: getattr(st, k) for k, _ in _ServerStats._fields_}

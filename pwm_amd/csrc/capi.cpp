@@ -72,6 +72,7 @@ const char* pwmhip_status_string(int status) {
     case PWMHIP_ERR_SYMBOL_RANGE: return "symbol does not fit into the requested levels";
     case PWMHIP_ERR_UNSUPPORTED: return "unsupported input (undefined in the reference)";
     case PWMHIP_ERR_BUDGET: return "request exceeds the device budget";
+    case PWMHIP_ERR_TIMEOUT: return "wait timed out";
     default: return "unknown status";
   }
 }

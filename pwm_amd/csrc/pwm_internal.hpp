@@ -85,6 +85,21 @@ struct pwmhip_ctx {
 
 namespace pwm {
 
+// One resident structure as the multi-structure query kernel reads it.
+struct QueryIndexDesc {
+  const void* lv;          // per-level rank/select descriptors (device)
+  const uint64_t* zeros;   // true zero counts (device)
+  uint64_t n;
+  uint32_t L;
+  int32_t tree;
+};
+QueryIndexDesc query_desc(const pwmhip_rs* rs);
+// Answers n symbol queries (op 0 access, 1 rank, 2 select; as pwmhip_query)
+// over several structures: query i reads d_desc[d_qidx[i]].  All device
+// pointers; enqueued on stream s.
+void launch_multi_query(hipStream_t s, const QueryIndexDesc* d_desc, const uint32_t* d_qidx, const uint32_t* op,
+                        const uint64_t* a, const uint64_t* b, uint64_t* res, uint64_t n);
+
 // The context's compress table, computed on first use.
 const uint8_t* compress_table(pwmhip_ctx* ctx);
 

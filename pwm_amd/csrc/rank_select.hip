@@ -232,24 +232,21 @@ __global__ void k_bv_query(QueryParams p) {
   p.res[i] = r;
 }
 
-// Symbol queries (query_support.hpp:72-178 recursions, iterative).
-__global__ void k_sym_query(QueryParams p) {
-  const uint64_t qi = uint64_t(blockIdx.x) * blockDim.x + threadIdx.x;
-  if (qi >= p.q) return;
-  const uint32_t op = p.op[qi];
-  const uint64_t a = p.a[qi], b = p.b[qi];
-  const uint32_t L = p.L;
+// Symbol queries (query_support.hpp:72-178 recursions, iterative) on one
+// structure: levels lv[0..L), true zero counts, text length n.
+__device__ uint64_t sym_query(const LevelRS* lv, uint32_t L, int tree, uint64_t n, const uint64_t* zeros,
+                              uint32_t op, uint64_t a, uint64_t b) {
   uint64_t res = kNone;
   if (op == 0) {  // access(a)
-    if (a < p.n) {
-      uint64_t sym = 0, i = a, s = 0, e = p.n;
+    if (a < n) {
+      uint64_t sym = 0, i = a, s = 0, e = n;
       for (uint32_t l = 0; l < L; ++l) {
-        const LevelRS& v = p.lv[l];
-        if (!p.tree) {
+        const LevelRS& v = lv[l];
+        if (!tree) {
           const bool bit = bit_at(v, i);
           sym = (sym << 1) | bit;
           const uint64_t r1 = rank1(v, i);
-          i = bit ? p.zeros[l] + r1 : i - r1;
+          i = bit ? zeros[l] + r1 : i - r1;
         } else {
           const uint64_t r1s = rank1(v, s), r1e = rank1(v, e), r1i = rank1(v, s + i);
           const uint64_t zn = (e - s) - (r1e - r1s);
@@ -265,14 +262,14 @@ __global__ void k_sym_query(QueryParams p) {
     const uint64_t c = a;
     if (L < 64 && (c >> L) != 0) {
       res = op == 1 ? 0 : (b == 0 ? 0 : kNone);
-    } else if (!p.tree) {
+    } else if (!tree) {
       // rank: (index, offset) pair (rank_matrix); select walks back up
-      uint64_t idx = op == 1 ? min(b, p.n) : p.n, off = 0;
+      uint64_t idx = op == 1 ? min(b, n) : n, off = 0;
       for (uint32_t l = 0; l < L; ++l) {
-        const LevelRS& v = p.lv[l];
+        const LevelRS& v = lv[l];
         const bool bit = (c >> (L - 1 - l)) & 1;
         const uint64_t ri = rank1(v, idx), ro = rank1(v, off);
-        if (bit) { idx = p.zeros[l] + ri; off = p.zeros[l] + ro; }
+        if (bit) { idx = zeros[l] + ri; off = zeros[l] + ro; }
         else { idx = idx - ri; off = off - ro; }
       }
       if (op == 1) res = idx - off;
@@ -280,18 +277,18 @@ __global__ void k_sym_query(QueryParams p) {
       else if (b <= idx - off) {
         uint64_t q = off + b - 1;  // position among the last level's successors
         for (int l = int(L) - 1; l >= 0; --l) {
-          const LevelRS& v = p.lv[l];
+          const LevelRS& v = lv[l];
           const bool bit = (c >> (L - 1 - l)) & 1;
-          q = bit ? select_k<1>(v, q - p.zeros[l] + 1) : select_k<0>(v, q + 1);
+          q = bit ? select_k<1>(v, q - zeros[l] + 1) : select_k<0>(v, q + 1);
         }
         res = q;
       }
     } else {
       // tree: node ranges down the path of c
       uint64_t st[PWMHIP_MAX_LEVELS];
-      uint64_t s = 0, e = p.n, i = op == 1 ? min(b, p.n) : 0;
+      uint64_t s = 0, e = n, i = op == 1 ? min(b, n) : 0;
       for (uint32_t l = 0; l < L; ++l) {
-        const LevelRS& v = p.lv[l];
+        const LevelRS& v = lv[l];
         st[l] = s;
         const bool bit = (c >> (L - 1 - l)) & 1;
         const uint64_t r1s = rank1(v, s), r1e = rank1(v, e), r1i = rank1(v, s + i);
@@ -304,7 +301,7 @@ __global__ void k_sym_query(QueryParams p) {
       else if (b <= e - s) {
         uint64_t q = b - 1;  // position inside the leaf
         for (int l = int(L) - 1; l >= 0; --l) {
-          const LevelRS& v = p.lv[l];
+          const LevelRS& v = lv[l];
           const bool bit = (c >> (L - 1 - l)) & 1;
           const uint64_t s0 = st[l];
           q = bit ? select_k<1>(v, rank1(v, s0) + q + 1) - s0 : select_k<0>(v, (s0 - rank1(v, s0)) + q + 1) - s0;
@@ -313,7 +310,22 @@ __global__ void k_sym_query(QueryParams p) {
       }
     }
   }
-  p.res[qi] = res;
+  return res;
+}
+
+__global__ void k_sym_query(QueryParams p) {
+  const uint64_t qi = uint64_t(blockIdx.x) * blockDim.x + threadIdx.x;
+  if (qi >= p.q) return;
+  p.res[qi] = sym_query(p.lv, p.L, p.tree, p.n, p.zeros, p.op[qi], p.a[qi], p.b[qi]);
+}
+
+// Queries over many resident structures: query i reads structure qidx[i].
+__global__ void k_multi_query(const QueryIndexDesc* desc, const uint32_t* qidx, const uint32_t* op,
+                              const uint64_t* a, const uint64_t* b, uint64_t* res, uint64_t q) {
+  const uint64_t qi = uint64_t(blockIdx.x) * blockDim.x + threadIdx.x;
+  if (qi >= q) return;
+  const QueryIndexDesc d = desc[qidx[qi]];
+  res[qi] = sym_query(static_cast<const LevelRS*>(d.lv), d.L, d.tree, d.n, d.zeros, op[qi], a[qi], b[qi]);
 }
 
 }  // namespace
@@ -321,6 +333,30 @@ __global__ void k_sym_query(QueryParams p) {
 }  // namespace pwm
 
 using namespace pwm;
+
+namespace pwm {
+
+QueryIndexDesc query_desc(const pwmhip_rs* rs) {
+  QueryIndexDesc d{};
+  d.lv = rs->d_desc;
+  d.zeros = reinterpret_cast<const uint64_t*>(static_cast<const char*>(rs->d_desc) +
+                                              ((rs->levels * sizeof(LevelRS) + 255) & ~size_t(255)));
+  d.n = rs->n;
+  d.L = uint32_t(rs->levels);
+  d.tree = rs->is_tree;
+  return d;
+}
+
+void launch_multi_query(hipStream_t s, const QueryIndexDesc* d_desc, const uint32_t* d_qidx, const uint32_t* op,
+                        const uint64_t* a, const uint64_t* b, uint64_t* res, uint64_t n) {
+  if (!n) return;
+  hipLaunchKernelGGL(k_multi_query, dim3(uint32_t((n + 255) / 256)), dim3(256), 0, s, d_desc, d_qidx, op, a, b,
+                     res, n);
+  PWM_HIP_TRY(hipGetLastError());
+}
+
+}  // namespace pwm
+
 
 extern "C" {
 
