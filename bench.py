@@ -36,7 +36,10 @@ CONFIGS = {
     "uniform64m_wm": ("uniform", 0xC1, 1 << 26, 1, 8, "ppc", False),
     "uniform64m_wt": ("uniform", 0xC1, 1 << 26, 1, 8, "pps", True),
     "zipf1g_wm": ("zipf", 0xC3, 1 << 28, 4, 20, "pps", False),
+    "dna4g_huffwm": ("dna", 0xC2, 1 << 32, 1, 0, "huff_pc", False),
 }
+# random symbol queries timed after the build
+QUERIES = {"zipf1g_wm": 10**7}
 
 
 # Many uneven texts, built together: (seed, texts, median length, sigma,
@@ -46,12 +49,19 @@ BATCH_CONFIGS = {
 }
 
 
-def cpu_baseline(text, levels, tree, width, sample_bytes):
-    """Reference algorithms (wx_ppc / wx_pps / wx_dd_pc) on a prefix sample."""
+def cpu_baseline(text, levels, tree, width, sample_bytes, algo="ppc"):
+    """Reference algorithms (wx_ppc / wx_pps / wx_dd_pc, or wx_huff_pc for a
+    Huffman-shaped build) on a prefix sample."""
     sys.path.insert(0, os.path.join(ROOT, "tests"))
     import oracle_lib
     cores = len(os.sched_getaffinity(0))
     sample = np.ascontiguousarray(text[: sample_bytes // width])
+    if oracle_lib.have_ref() and algo.startswith("huff"):
+        oracle_lib.ref_build("huff_pc", sample[: 1 << 20], 0, tree)  # warm-up
+        ts = sorted(oracle_lib.ref_build("huff_pc", sample, 0, tree)[1] for _ in range(3))
+        return {"value": round(sample.nbytes / ts[1] / 1e9, 3), "unit": "GB/s", "cores": 1, "kind": "reference",
+                "sample": "%d MiB prefix of the same text; wx_huff_pc (sequential in the reference), "
+                          "median of 3" % (sample.nbytes >> 20)}
     if oracle_lib.have_ref():
         os.environ.setdefault("OMP_NUM_THREADS", str(cores))
         best = None
@@ -68,7 +78,10 @@ def cpu_baseline(text, levels, tree, width, sample_bytes):
                           % (sample.nbytes >> 20, best[0], per)}
     small = sample[: 16 << 20]
     t0 = time.perf_counter()
-    oracle_lib.naive(small, levels, tree)
+    if algo.startswith("huff"):
+        oracle_lib.huff_pc(small, tree)
+    else:
+        oracle_lib.naive(small, levels, tree)
     dt = time.perf_counter() - t0
     return {"value": round(small.nbytes / dt / 1e9, 3), "unit": "GB/s", "cores": 1, "kind": "port",
             "sample": "%d MiB prefix, scalar CPU restatement (reference build absent)" % (small.nbytes >> 20)}
@@ -254,6 +267,109 @@ def run_batch(args):
     print(json.dumps(out), flush=True)
 
 
+# Serving: Poisson arrivals of build+query requests (seed, requests per rate,
+# median length, sigma, min length, max length, queries per request, budget)
+SERVE_CONFIGS = {
+    "serve": (0xC5, 200, 1 << 20, 2.0, 4 << 10, 256 << 20, 100_000, 200 * 10**9),
+}
+
+
+def run_serve(args):
+    """Serving loop: saturation throughput, then latency at fractions of it."""
+    import threading
+    import torch
+    import pwm_amd
+
+    seed, nreq, median, sigma, lo, hi, nq, budget = SERVE_CONFIGS[args.config]
+    rng = np.random.default_rng(seed)
+    pool_len = 512 << 20
+    # host text pools in pinned memory, one per alphabet (5, 20, 256 symbols)
+    pools = []
+    for kind, sig in (("dna", 5), ("u20", 20), ("u256", 256)):
+        buf = torch.empty(pool_len, dtype=torch.uint8, pin_memory=True)
+        if kind == "dna":
+            buf.numpy()[:] = pwm_amd.generate("dna", seed, pool_len)
+        else:
+            buf.numpy()[:] = np.random.default_rng(seed + sig).integers(0, sig, pool_len, dtype=np.uint8)
+        pools.append((buf, sig, {5: 3, 20: 5, 256: 8}[sig]))
+
+    def make_requests(r):
+        reqs = []
+        for _ in range(nreq):
+            buf, sig, L = pools[r.choice(3, p=[0.6, 0.3, 0.1])]
+            n = int(np.clip(r.lognormal(np.log(median), sigma), lo, hi))
+            off = int(r.integers(0, pool_len - n + 1))
+            text = buf.numpy()[off:off + n]
+            op = r.integers(0, 3, nq).astype(np.uint32)
+            pos = r.integers(0, n, nq)
+            a = np.where(op == 0, pos, text[r.integers(0, n, nq)]).astype(np.uint64)
+            b = np.where(op == 1, r.integers(0, n + 1, nq), r.integers(1, 4, nq)).astype(np.uint64)
+            reqs.append((text, L, op, a, b))
+        return reqs
+
+    def run(reqs, gaps):
+        srv = pwm_amd.Server(hbm_budget_bytes=budget, workers=3)
+        ids = []
+
+        def submit():
+            for (text, L, op, a, b), g in zip(reqs, gaps):
+                if g > 0:
+                    time.sleep(g)
+                bid = srv.build(text, L, "ppc")
+                ids.append((bid, srv.query(bid, op, a, b)))
+
+        t0 = time.perf_counter()
+        th = threading.Thread(target=submit)
+        th.start()
+        th.join()
+        for bid, qid in ids:
+            srv.wait(qid)
+        wall = time.perf_counter() - t0
+        for bid, _ in ids:
+            srv.release(bid)
+        st = srv.stats()
+        srv.close()
+        return wall, st
+
+    reqs = make_requests(rng)
+    total_bytes = sum(r[0].size for r in reqs)
+    run(reqs[:20], [0.0] * 20)  # warm-up
+    wall, st = run(reqs, [0.0] * nreq)  # all at once: saturation
+    sat_rps = nreq / wall
+    sweep = []
+    for frac in (0.25, 0.5, 0.75, 1.0):
+        lam = frac * sat_rps
+        gaps = rng.exponential(1.0 / lam, nreq)
+        w, s2 = run(reqs, gaps)
+        sweep.append({"offered_rps": round(lam, 1), "achieved_rps": round(nreq / w, 1),
+                      "request_p50_ms": round(s2["query_p50_ms"], 3), "request_p99_ms": round(s2["query_p99_ms"], 3),
+                      "build_p50_ms": round(s2["build_p50_ms"], 3), "build_p99_ms": round(s2["build_p99_ms"], 3),
+                      "texts_per_launch": round(s2["mean_texts_per_launch"], 2),
+                      "peak_budget_bytes": int(s2["bytes_peak"])})
+    out = {
+        "metric": "WM/WT build input GB/s + % HBM peak; batched vs per-text launches, bit-exact",
+        "value": round(total_bytes / wall / 1e9, 3),
+        "unit": "GB/s",
+        "n_gpus": 1, "steps": 1, "warmup": 1,
+        "ms_per_step": round(wall * 1e3, 3),
+        "higher_is_better": True, "scaling": "weak", "vs_baseline": None, "dtype": "u8",
+        "data": "synthetic (dna / uniform-20 / uniform-256 pools, seed 0x%X, pinned host memory)" % seed,
+        "config": {"workload": "%s: %d requests, lengths lognormal(median %d, sigma %.1f) clipped [%d, %d], "
+                               "alphabets 5/20/256 (p 0.6/0.3/0.1; levels 3/5/8), wm_ppc, %d queries each, "
+                               "HBM budget %d B, 3 worker streams" % (args.config, nreq, median, sigma, lo, hi, nq,
+                                                                       budget),
+                   "requests": nreq, "text_bytes": int(total_bytes),
+                   "saturation_rps": round(sat_rps, 1),
+                   "saturation_request_p50_ms": round(st["query_p50_ms"], 3),
+                   "saturation_request_p99_ms": round(st["query_p99_ms"], 3),
+                   "queries_per_s": round(nreq * nq / wall, 1),
+                   "texts_per_launch": round(st["mean_texts_per_launch"], 2),
+                   "sweep": sweep},
+        "roofline": None,
+    }
+    print(json.dumps(out), flush=True)
+
+
 def run_copy(args):
     """A plain copy of known size: its counter readings per byte scale the
     construction kernels' readings."""
@@ -278,9 +394,11 @@ def main():
     ap.add_argument("--gpus", type=int, default=1)
     ap.add_argument("--steps", type=int, default=10)
     ap.add_argument("--warmup", type=int, default=3)
-    ap.add_argument("--config", default="dna4g_wm", choices=sorted(CONFIGS) + sorted(BATCH_CONFIGS))
+    ap.add_argument("--config", default="dna4g_wm",
+                    choices=sorted(CONFIGS) + sorted(BATCH_CONFIGS) + sorted(SERVE_CONFIGS))
     ap.add_argument("--n", type=int, default=0, help="override the text length")
     ap.add_argument("--no-cpu", action="store_true")
+    ap.add_argument("--queries", type=int, default=None, help="random symbol queries to time after the build")
     ap.add_argument("--no-per-text", action="store_true",
                     help="many-texts configs: skip the one-build-per-text comparison")
     ap.add_argument("--cpu-sample-mib", type=int, default=256)
@@ -298,6 +416,9 @@ def main():
         return
     if args.config in BATCH_CONFIGS:
         run_batch(args)
+        return
+    if args.config in SERVE_CONFIGS:
+        run_serve(args)
         return
 
     import torch
@@ -331,13 +452,14 @@ def main():
 
     # light self-check of the last build: zeros against a host histogram
     z = ws.zeros()
-    if not tree:
+    if not tree and levels:
         hist = np.bincount(text[: min(n, 1 << 28)], minlength=1 << levels) if width == 1 and n <= (1 << 28) else None
         if hist is not None:
             exp0 = int(hist[: 1 << (levels - 1)].sum())
             assert int(z[0]) == exp0, "zeros[0] mismatch"
 
-    b_min = n * width + levels * 8 * ((n + 63) // 64)
+    # text read once + every level's words written once (Huffman levels are ragged)
+    b_min = n * width + 8 * sum((ws.level_bit_size(l) + 63) // 64 for l in range(ws.levels()))
     roof, whole = rooflines(b_min, phases, dev_total, args.config)
     value = n * width / (ms_per_step / 1e3) / 1e9
     out = {
@@ -354,8 +476,8 @@ def main():
         "dtype": "u%d" % (8 * width),
         "data": "synthetic (%s generator, seed 0x%X)" % (kind, seed),
         "config": {"workload": "%s: %d x uint%d, %d levels, %s %s" % (
-            args.config, n, 8 * width, levels, "wt" if tree else "wm", algo),
-            "symbols": n, "levels": levels, "algorithm": ("wt_" if tree else "wm_") + algo,
+            args.config, n, 8 * width, ws.levels(), "wt" if tree else "wm", algo),
+            "symbols": n, "levels": ws.levels(), "algorithm": ("wt_" if tree else "wm_") + algo,
             "device_ms_per_build": round(dev_total, 4),
             "device_ms_phases": {"local_build": phases[0], "offsets": phases[1], "stitch": phases[2]},
             "symbols_per_s": round(n / (ms_per_step / 1e3), 1),
@@ -363,11 +485,79 @@ def main():
             "generate_s": round(gen_s, 2)},
         "roofline": roof,
     }
+    nq = args.queries if args.queries is not None else QUERIES.get(args.config, 0)
+    if nq:
+        out["config"]["queries"] = time_queries(pwm_amd, ctx, ws, text, nq)
     if not args.no_cpu:
-        out["cpu_baseline"] = cpu_baseline(text, levels, tree, width, args.cpu_sample_mib << 20)
+        out["cpu_baseline"] = cpu_baseline(text, levels, tree, width, args.cpu_sample_mib << 20, algo)
     ws.free()
     ctx.close()
     print(json.dumps(out), flush=True)
+
+
+def time_queries(pwm_amd, ctx, ws, text, nq, checked=20000):
+    """Random access / rank / select over the built structure: throughput of
+    one batch of nq queries with device-resident arguments, and a bit-exact
+    check of a sample against positions computed on the host."""
+    import ctypes
+    import torch
+    rng = np.random.default_rng(0xC3)
+    n = text.size
+    qs = pwm_amd.QuerySupport(ws)
+    op = rng.integers(0, 3, nq).astype(np.uint32)
+    pos = rng.integers(0, n, nq)
+    sym = text[rng.integers(0, n, nq)].astype(np.uint64)
+    a = np.where(op == 0, pos, sym).astype(np.uint64)
+    b = np.where(op == 1, rng.integers(0, n + 1, nq), rng.integers(1, 3, nq)).astype(np.uint64)
+    dev = torch.device("cuda:0")
+    d_op = torch.from_numpy(op.view(np.int32)).to(dev)
+    d_a = torch.from_numpy(a.view(np.int64)).to(dev)
+    d_b = torch.from_numpy(b.view(np.int64)).to(dev)
+    d_r = torch.empty(nq, dtype=torch.int64, device=dev)
+    lib = pwm_amd.lib()
+
+    def go():
+        st = lib.pwmhip_query(ctx.h, ctypes.byref(ws._ws), qs.h, ctypes.c_void_p(d_op.data_ptr()),
+                              ctypes.c_void_p(d_a.data_ptr()), ctypes.c_void_p(d_b.data_ptr()), nq,
+                              ctypes.c_void_p(d_r.data_ptr()), pwm_amd.FLAG_TEXT_ON_DEVICE)
+        if st:
+            raise pwm_amd.PwmError(st, "query")
+
+    go()
+    torch.cuda.synchronize()
+    t0 = time.perf_counter()
+    reps = 3
+    for _ in range(reps):
+        go()
+    torch.cuda.synchronize()
+    dt = (time.perf_counter() - t0) / reps
+    res = d_r.cpu().numpy().view(np.uint64)
+    acc = op == 0
+    ok = bool(np.array_equal(res[acc], text[a[acc]].astype(np.uint64)))
+    # rank / select checked on a second batch over 16 symbols of the text,
+    # against their positions found on the host
+    syms = np.unique(text[rng.integers(0, n, 16)])
+    occ = {int(c): np.flatnonzero(text == c) for c in syms}
+    m = checked
+    op2 = rng.integers(1, 3, m).astype(np.uint32)
+    a2 = syms[rng.integers(0, syms.size, m)].astype(np.uint64)
+    b2 = np.empty(m, np.uint64)
+    want = np.empty(m, np.uint64)
+    for k in range(m):
+        p_ = occ[int(a2[k])]
+        if op2[k] == 1:
+            b2[k] = rng.integers(0, n + 1)
+            want[k] = np.searchsorted(p_, b2[k], "left")
+        else:
+            b2[k] = rng.integers(1, p_.size + 2)
+            want[k] = p_[b2[k] - 1] if b2[k] <= p_.size else np.uint64(2**64 - 1)
+    got = qs.query(op2, a2, b2)
+    ok = ok and bool(np.array_equal(got, want))
+    qs.free()
+    return {"count": nq, "ms": round(dt * 1e3, 3), "queries_per_s": round(nq / dt, 1),
+            "mix": "access/rank/select uniform; positions uniform; symbols drawn from the text",
+            "checked": "%d access in the timed batch + %d rank/select over 16 symbols" % (int(acc.sum()), m),
+            "bit_exact": ok}
 
 
 if __name__ == "__main__":

@@ -44,12 +44,28 @@ __global__ void k_copy_bits(uint64_t* dst, const uint64_t* src, uint64_t dst_off
   else dst[w] |= v;  // at most the first and the last word; distinct threads
 }
 
+// Leaf histogram.  Up to 2^13 leaves the block counts in LDS first (global
+// atomics on few addresses serialise); above that it adds in global memory.
+constexpr uint32_t kLdsLeaves = 1u << 13;
+
 template <typename T>
-__global__ void k_leaf_hist(const T* text, uint64_t n, uint32_t L, unsigned long long* hist) {
+__global__ __launch_bounds__(256) void k_leaf_hist(const T* text, uint64_t n, uint32_t L,
+                                                   unsigned long long* hist) {
+  __shared__ uint32_t lh[kLdsLeaves];
+  const bool local = L <= 13;
+  const uint32_t K = local ? (1u << L) : 0u;
+  for (uint32_t i = threadIdx.x; i < K; i += blockDim.x) lh[i] = 0;
+  __syncthreads();
   for (uint64_t i = uint64_t(blockIdx.x) * blockDim.x + threadIdx.x; i < n; i += uint64_t(gridDim.x) * blockDim.x) {
     const uint64_t c = text[i];
-    if (L >= 64 || (c >> L) == 0) atomicAdd(&hist[c], 1ull);
+    if (L >= 64 || (c >> L) == 0) {
+      if (local) atomicAdd(&lh[c], 1u);
+      else atomicAdd(&hist[c], 1ull);
+    }
   }
+  __syncthreads();
+  for (uint32_t i = threadIdx.x; i < K; i += blockDim.x)
+    if (lh[i]) atomicAdd(&hist[i], (unsigned long long)lh[i]);
 }
 
 // One thread per destination word of a tree level; the word gathers the bits

@@ -384,6 +384,8 @@ __global__ __launch_bounds__(64 * waves_for<NP>()) void k_local_build(K1Params p
   constexpr int WAVES = waves_for<NP>();
   __shared__ __align__(16) uint8_t lut[65536];
   __shared__ WaveLds<NP> wl[WAVES];
+  __shared__ unsigned long long red[kMaxGroup + 1];  // block sums: zeros per sub-level, symbol OR
+  if (threadIdx.x <= kMaxGroup) red[threadIdx.x] = 0;
 
   const int lane = threadIdx.x & 63;
   const int wid = threadIdx.x >> 6;
@@ -565,11 +567,23 @@ __global__ __launch_bounds__(64 * waves_for<NP>()) void k_local_build(K1Params p
     c = cn;
     cv = nv;
   }
-  flush_zeros();
-  if (!p.planes_in) {
+  // one text: the block's waves add up in LDS, then one global atomic per
+  // counter per block (thousands of same-address atomics serialise)
+  if (p.b_chunk_base) flush_zeros();
+  else if (lane == 0) {
 #pragma unroll
-    for (int d = 32; d >= 1; d >>= 1) my_or |= __shfl_xor(my_or, d, 64);
-    if (lane == 0 && my_or) atomicOr(p.sym_or, (unsigned long long)my_or);
+    for (int j = 0; j < kZ; ++j)
+      if (j < int(p.g) && zsum[j]) atomicAdd(&red[j], (unsigned long long)zsum[j]);
+  }
+#pragma unroll
+  for (int d = 32; d >= 1; d >>= 1) my_or |= __shfl_xor(my_or, d, 64);
+  if (lane == 0 && my_or) atomicOr(&red[kMaxGroup], (unsigned long long)my_or);
+  __syncthreads();
+  if (threadIdx.x == 0) {
+    if (!p.b_chunk_base)
+      for (int j = 0; j < kZ; ++j)
+        if (red[j]) atomicAdd(p.zeros_out + j, red[j]);
+    if (!p.planes_in && red[kMaxGroup]) atomicOr(p.sym_or, red[kMaxGroup]);
   }
 }
 
@@ -601,15 +615,21 @@ __device__ __forceinline__ uint64_t compress64h(uint64_t x, uint64_t m, const ui
   return uint64_t(lo) | (uint64_t(hi) << __builtin_popcount(uint32_t(m)));
 }
 
-template <int G>
+// With SC (a pass followed by a symbol reorder) the bucket counts of the
+// order after all G levels (2^G buckets, the next pass's level order) are
+// produced as well, as "sub-level G".
+template <int G, int SC>
 __global__ __launch_bounds__(64 * 16) void k_local_small(K1Params p) {
   constexpr int WAVES = 16;
-  constexpr int NSUB = G > 1 ? G - 1 : 1;  // sub-levels 1..G-1
-  constexpr int NQ = 1 << (G - 1);          // buckets of the last sub-level
-  constexpr int NF = NQ + 1;                // scan fields: bucket counts + last level's zeros
-  constexpr int NS = (NF + 1) / 2;          // packed scans
+  constexpr int NSUB = G > 1 ? G - 1 : 1;  // sub-levels 1..G-1 (bits assembled)
+  constexpr int JC = G - 1 + SC;           // last sub-level with bucket counts
+  constexpr int NQ = 1 << JC;              // buckets of that sub-level
+  constexpr int NF = NQ + (SC ? 0 : 1);    // scan fields: bucket counts (+ last level's zeros)
+  constexpr int NS = (NF + 1) / 2;         // packed scans
   __shared__ __align__(16) uint8_t lut[65536];
   __shared__ uint64_t sbuf[WAVES][NSUB][64];
+  __shared__ unsigned long long red[G + 1];  // block sums: zeros per level, symbol OR
+  if (threadIdx.x <= G) red[threadIdx.x] = 0;
 
   const int lane = threadIdx.x & 63;
   const int wid = threadIdx.x >> 6;
@@ -704,11 +724,11 @@ __global__ __launch_bounds__(64 * 16) void k_local_small(K1Params p) {
 #pragma unroll
     for (int j = 0; j < NSUB; ++j) buf[j][lane] = 0;
 
-    // bucket masks of the last sub-level, M[q] = prod_i (bit i of q ? P_i : ~P_i)
+    // bucket masks of sub-level JC, M[q] = prod_i (bit i of q ? P_i : ~P_i)
     uint64_t M[NQ];
     M[0] = vm;
 #pragma unroll
-    for (int i = 0; i < G - 1; ++i) {
+    for (int i = 0; i < JC; ++i) {
 #pragma unroll
       for (int q = (1 << i) - 1; q >= 0; --q) {
         M[q + (1 << i)] = M[q] & P0[i];
@@ -718,7 +738,7 @@ __global__ __launch_bounds__(64 * 16) void k_local_small(K1Params p) {
     uint32_t f[2 * NS];
 #pragma unroll
     for (int q = 0; q < NQ; ++q) f[q] = __popcll(M[q]);
-    f[NQ] = __popcll(~P0[G - 1] & vm);
+    if (!SC) f[NQ] = __popcll(~P0[G - 1] & vm);
     if (NF < 2 * NS) f[2 * NS - 1] = 0;
     uint32_t ex[2 * NS], tot[2 * NS];
 #pragma unroll
@@ -736,21 +756,21 @@ __global__ __launch_bounds__(64 * 16) void k_local_small(K1Params p) {
     // of sub-level j+1, i.e. q with bit j clear among the last sub-level's
     {
 #pragma unroll
-      for (int j = 0; j < G - 1; ++j) {
+      for (int j = 0; j < JC; ++j) {
         uint32_t z = 0;
 #pragma unroll
         for (int q = 0; q < NQ; ++q)
           if (!((q >> j) & 1)) z += tot[q];
         zsum[j] += z;
       }
-      zsum[G - 1] += tot[NQ];
+      if (!SC) zsum[G - 1] += tot[NQ];
     }
 
     // sub-level j (1..G-1): buckets q < 2^j; counts/prefixes are sums over
     // the last sub-level's buckets that extend q
     uint32_t cslot[NQ > 1 ? 2 * NQ - 2 : 1];
 #pragma unroll
-    for (int j = 1; j < G; ++j) {
+    for (int j = 1; j <= JC; ++j) {
       const int nb = 1 << j;
       uint32_t start = 0;
 #pragma unroll
@@ -761,11 +781,13 @@ __global__ __launch_bounds__(64 * 16) void k_local_small(K1Params p) {
           T += tot[r];
           E += ex[r];
         }
-        uint64_t mq = vm;
+        if (j < G) {
+          uint64_t mq = vm;
 #pragma unroll
-        for (int i = 0; i < j; ++i) mq &= ((q >> i) & 1) ? P0[i] : ~P0[i];
-        const uint32_t len = __popcll(mq);
-        or_bits(buf[j - 1], start + E, compress64h(P0[j], mq, lut), len);
+          for (int i = 0; i < j; ++i) mq &= ((q >> i) & 1) ? P0[i] : ~P0[i];
+          const uint32_t len = __popcll(mq);
+          or_bits(buf[j - 1], start + E, compress64h(P0[j], mq, lut), len);
+        }
         cslot[nb - 2 + q] = T;
         start += T;
       }
@@ -785,10 +807,115 @@ __global__ __launch_bounds__(64 * 16) void k_local_small(K1Params p) {
     c = cn;
     cv = nv;
   }
-  flush_zeros();
+  // one text: the block's waves add up in LDS, then one global atomic per
+  // counter per block (thousands of same-address atomics serialise)
+  if (p.b_chunk_base) flush_zeros();
+  else if (lane == 0) {
+#pragma unroll
+    for (int j = 0; j < G; ++j)
+      if (zsum[j]) atomicAdd(&red[j], (unsigned long long)zsum[j]);
+  }
 #pragma unroll
   for (int d = 32; d >= 1; d >>= 1) my_or |= __shfl_xor(my_or, d, 64);
-  if (lane == 0 && my_or) atomicOr(p.sym_or, (unsigned long long)my_or);
+  if (lane == 0 && my_or) atomicOr(&red[G], (unsigned long long)my_or);
+  __syncthreads();
+  if (threadIdx.x == 0) {
+    if (!p.b_chunk_base)
+      for (int j = 0; j < G; ++j)
+        if (red[j]) atomicAdd(p.zeros_out + j, red[j]);
+    if (red[G]) atomicOr(p.sym_or, red[G]);
+  }
+}
+
+// ---- symbol reorder between passes ------------------------------------------
+// After a pass over levels [l0, l0+G), the next pass reads the symbols in the
+// order of level l0+G: stable by bucket q = (bit of level l0+G-1 ... bit of
+// level l0), q ascending.  One wave per chunk: per-lane bucket counts in LDS,
+// wave scans for lane prefixes, the chunk's symbols placed in bucket order in
+// LDS, then written out so that each bucket's run is a contiguous store.
+struct ScatterParams {
+  const void* in;
+  void* out;
+  uint64_t n;
+  uint32_t sh;               // the G-bit field of the pass's levels starts at bit sh
+  const uint64_t* dstpos;    // [c * 2^G + q]: global position of (chunk, bucket)
+  uint64_t nchunks;
+};
+
+template <typename T, int G>
+__global__ __launch_bounds__(256) void k_scatter(ScatterParams p) {
+  constexpr int WAVES = 4;
+  constexpr int NQ = 1 << G;
+  constexpr int CS = NQ + 1;  // per-lane counter stride (no bank conflicts)
+  __shared__ uint16_t stage[WAVES][kChunkSyms];  // chunk index of each sorted position
+  __shared__ uint8_t kbuf[WAVES][kChunkSyms];     // bucket of each symbol
+  __shared__ uint32_t cnt[WAVES][64 * CS];
+  __shared__ uint64_t offs[WAVES][NQ];
+  const int lane = threadIdx.x & 63, wid = threadIdx.x >> 6;
+  const uint64_t c = uint64_t(blockIdx.x) * WAVES + wid;
+  if (c >= p.nchunks) return;  // whole waves; no workgroup barrier below
+  const auto in = gmem(static_cast<const T*>(p.in) + c * kChunkSyms);
+  const uint32_t E = uint32_t(min<uint64_t>(kChunkSyms, p.n - c * kChunkSyms));
+  const int m = max(0, min(64, int(E) - 64 * lane));
+  auto key = [&](uint64_t x) -> uint32_t {
+    const uint32_t f = uint32_t(x >> p.sh) & uint32_t(NQ - 1);
+    return __builtin_bitreverse32(f) >> (32 - G);
+  };
+  // keys of the lane's symbols into LDS (loads issued 16 at a time)
+  uint8_t* keys = &kbuf[wid][64 * lane];
+  const T* mine = static_cast<const T*>(p.in) + c * kChunkSyms + 64 * lane;
+#pragma unroll
+  for (int blk = 0; blk < 4; ++blk) {
+    T v[16];
+#pragma unroll
+    for (int e = 0; e < 16; ++e) v[e] = (16 * blk + e < m) ? gmem(mine)[16 * blk + e] : T(0);
+#pragma unroll
+    for (int e = 0; e < 16; ++e) keys[16 * blk + e] = uint8_t(key(uint64_t(v[e])));
+  }
+  uint32_t* my = &cnt[wid][lane * CS];
+#pragma unroll
+  for (int q = 0; q < NQ; ++q) my[q] = 0;
+  for (int i = 0; i < m; ++i) ++my[keys[i]];
+  // lane prefixes and chunk totals per bucket (two 16-bit fields per scan)
+  // (each pair of counters is read before it is replaced by its start)
+  uint32_t run = 0;
+#pragma unroll
+  for (int q = 0; q < NQ; q += 2) {
+    const uint32_t v = my[q] | (q + 1 < NQ ? my[q + 1] << 16 : 0u);
+    const uint32_t inc = wave_incl_scan(v);
+    const uint32_t t = __builtin_amdgcn_readlane(inc, 63);
+    const uint32_t e = inc - v;
+    my[q] = run + (e & 0xffffu);
+    if (lane == 0) offs[wid][q] = p.dstpos[c * NQ + q] - run;
+    run += t & 0xffffu;
+    if (q + 1 < NQ) {
+      my[q + 1] = run + (e >> 16);
+      if (lane == 0) offs[wid][q + 1] = p.dstpos[c * NQ + q + 1] - run;
+      run += t >> 16;
+    }
+  }
+  for (int i = 0; i < m; ++i) {
+    const uint32_t q = keys[i];
+    const uint32_t pos = my[q];
+    my[q] = pos + 1;
+    stage[wid][pos] = uint16_t(64 * lane + i);
+  }
+  wave_sync();
+  const auto out = gmem(static_cast<T*>(p.out));
+  // each bucket's run is a contiguous store; four symbols in flight per lane
+  for (uint32_t k0 = 0; k0 < E; k0 += 256) {
+    T x[4];
+#pragma unroll
+    for (int u = 0; u < 4; ++u) {
+      const uint32_t k = k0 + 64 * u + lane;
+      x[u] = k < E ? in[stage[wid][k]] : T(0);
+    }
+#pragma unroll
+    for (int u = 0; u < 4; ++u) {
+      const uint32_t k = k0 + 64 * u + lane;
+      if (k < E) out[offs[wid][key(uint64_t(x[u]))] + k] = x[u];
+    }
+  }
 }
 
 // ---- K2 --------------------------------------------------------------------
@@ -1003,9 +1130,10 @@ struct GatherParams {
   uint64_t dst_stride;
   uint32_t nplanes;
   uint64_t n;
+  uint32_t nwin;            // 64-word windows per wave (multiple of kGroup)
 };
 
-constexpr uint32_t kGatherWindows = 32;  // 64-word windows per wave
+constexpr uint32_t kGatherWindows = 32;  // 64-word windows per wave (at most)
 constexpr uint32_t kGroup = 4;           // windows whose loads are issued together
 
 __device__ __forceinline__ uint64_t readlane64(uint64_t v, uint32_t lane) {
@@ -1209,9 +1337,9 @@ __device__ __forceinline__ void gather_range(const uint64_t* tab_dst, const uint
 }
 
 __global__ __launch_bounds__(256) void k_gather(GatherParams p) {
-  const uint64_t W0 = (uint64_t(blockIdx.x) * 4 + (threadIdx.x >> 6)) * 64 * kGatherWindows;
+  const uint64_t W0 = (uint64_t(blockIdx.x) * 4 + (threadIdx.x >> 6)) * 64 * p.nwin;
   gather_range(p.tab_dst, p.tab_src, p.level_ne[p.j], p.src, p.src_stride, p.dst, p.dst_stride, p.nplanes, p.n,
-               W0, kGatherWindows);
+               W0, p.nwin);
 }
 
 // Batch: waves are assigned to texts by wave_base (per text), level j.
@@ -1282,10 +1410,14 @@ __global__ __launch_bounds__(256) void k_tile_sums2(const uint32_t* counts, uint
 
 // ... then (after the tile sums are scanned and the bucket starts known) every
 // non-empty (chunk, bucket) run is written into its level's table.
+// Slots from sc_slot0 on are the buckets of the order after the pass: their
+// global start per (chunk, bucket) goes to dstpos (for the symbol reorder)
+// instead of a run table.
 __global__ __launch_bounds__(1024) void k_apply_runs(const uint32_t* counts, const uint64_t* tsum,
                                                      const uint64_t* tnz, const uint64_t* gstart,
                                                      const uint64_t* nebase, uint64_t* tab_dst, uint64_t* tab_src,
-                                                     uint64_t nchunks, uint32_t nslot, uint64_t ntiles) {
+                                                     uint64_t nchunks, uint32_t nslot, uint64_t ntiles,
+                                                     uint64_t* dstpos, uint32_t sc_slot0) {
   __shared__ uint64_t wsum[2][16];
   const uint64_t tile = blockIdx.x;
   const uint32_t slot = blockIdx.y;
@@ -1315,6 +1447,10 @@ __global__ __launch_bounds__(1024) void k_apply_runs(const uint32_t* counts, con
       oz += wsum[1][w];
     }
   const uint64_t pref = ov + iv - x, nzpref = oz + iz - 1;
+  if (slot >= sc_slot0) {
+    dstpos[c * (nslot - sc_slot0) + (slot - sc_slot0)] = gstart[slot] + pref;
+    return;
+  }
   const uint32_t j = slot_level(slot);
   const uint32_t s0 = (1u << j) - 2;
   uint64_t loc = 0;  // chunk-local start: counts of the chunk's earlier buckets of this level
@@ -1441,22 +1577,33 @@ void launch_k1(pwmhip_ctx* ctx, const K1Params& prm) {
   PWM_HIP_TRY(hipGetLastError());
 }
 
-template <int G>
+template <int G, int SC>
 void launch_small(pwmhip_ctx* ctx, const K1Params& prm) {
   const uint64_t grid = std::min<uint64_t>((prm.nchunks + 15) / 16, uint64_t(ctx->num_cus));
-  hipLaunchKernelGGL(k_local_small<G>, dim3(uint32_t(grid)), dim3(64 * 16), 0, ctx->stream, prm);
+  hipLaunchKernelGGL((k_local_small<G, SC>), dim3(uint32_t(grid)), dim3(64 * 16), 0, ctx->stream, prm);
   PWM_HIP_TRY(hipGetLastError());
+}
+
+// A pass of g <= 4 levels from the text, with (sc) or without the bucket
+// counts of the order after it.
+void dispatch_small(pwmhip_ctx* ctx, const K1Params& prm, bool sc) {
+  switch (prm.g * 2 + (sc ? 1 : 0)) {
+    case 2: launch_small<1, 0>(ctx, prm); return;
+    case 3: launch_small<1, 1>(ctx, prm); return;
+    case 4: launch_small<2, 0>(ctx, prm); return;
+    case 5: launch_small<2, 1>(ctx, prm); return;
+    case 6: launch_small<3, 0>(ctx, prm); return;
+    case 7: launch_small<3, 1>(ctx, prm); return;
+    case 8: launch_small<4, 0>(ctx, prm); return;
+    default: launch_small<4, 1>(ctx, prm); return;
+  }
 }
 
 void dispatch_k1(pwmhip_ctx* ctx, const K1Params& prm) {
   const uint32_t np = prm.np;
   if (!prm.planes_in && prm.g == np && np <= 4 && !small_k1_disabled()) {
-    switch (np) {
-      case 1: launch_small<1>(ctx, prm); return;
-      case 2: launch_small<2>(ctx, prm); return;
-      case 3: launch_small<3>(ctx, prm); return;
-      default: launch_small<4>(ctx, prm); return;
-    }
+    dispatch_small(ctx, prm, false);
+    return;
   }
   if (np <= 1) launch_k1<1>(ctx, prm);
   else if (np <= 2) launch_k1<2>(ctx, prm);
@@ -1508,8 +1655,197 @@ const uint8_t* compress_table(pwmhip_ctx* ctx) {
   return ctx->d_lut;
 }
 
+namespace {
+
+// Windows per gather wave: enough waves to fill the chip (about 64 per CU).
+uint32_t gather_windows(uint64_t W, uint32_t num_cus) {
+  const uint64_t target = uint64_t(num_cus) * 64;
+  uint64_t nwin = (W + 64 * target - 1) / (64 * target);
+  nwin = std::min<uint64_t>(std::max<uint64_t>(nwin, kGroup), kGatherWindows);
+  return uint32_t((nwin + kGroup - 1) / kGroup * kGroup);
+}
+
+template <int G>
+void launch_scatter_g(hipStream_t s, uint32_t width, const ScatterParams& sp) {
+  const dim3 grid(uint32_t((sp.nchunks + 3) / 4)), block(256);
+  switch (width) {
+    case 1: hipLaunchKernelGGL((k_scatter<uint8_t, G>), grid, block, 0, s, sp); break;
+    case 2: hipLaunchKernelGGL((k_scatter<uint16_t, G>), grid, block, 0, s, sp); break;
+    case 4: hipLaunchKernelGGL((k_scatter<uint32_t, G>), grid, block, 0, s, sp); break;
+    default: hipLaunchKernelGGL((k_scatter<uint64_t, G>), grid, block, 0, s, sp); break;
+  }
+}
+
+void launch_scatter(hipStream_t s, uint32_t width, uint32_t g, const ScatterParams& sp) {
+  switch (g) {
+    case 1: launch_scatter_g<1>(s, width, sp); break;
+    case 2: launch_scatter_g<2>(s, width, sp); break;
+    case 3: launch_scatter_g<3>(s, width, sp); break;
+    default: launch_scatter_g<4>(s, width, sp); break;
+  }
+  PWM_HIP_TRY(hipGetLastError());
+}
+
+}  // namespace
+
+// Passes of at most 4 levels over the text; between passes the symbols are
+// reordered into the next pass's level order, so every pass sees few
+// buckets per chunk (long runs) whatever the number of levels.  Trees are
+// built as matrices and their buckets moved into numeric order at the end.
+void build_passes_device(pwmhip_ctx* ctx, const void* d_text, uint64_t n, uint32_t width, uint32_t L, bool tree,
+                         bool check_range, pwmhip_ws* ws) {
+  hipStream_t s = ctx->stream;
+  const uint64_t nchunks = (n + kChunkSyms - 1) / kChunkSyms;
+  const uint64_t stride = level_stride(n);
+  const uint32_t P = (L + 3) / 4;
+  std::vector<uint32_t> gs;
+  for (uint32_t i = 0; i < P; ++i) gs.push_back(L / P + (i < L % P ? 1 : 0));
+  const uint32_t gmax = gs[0];
+  const uint32_t nslot_max = (2u << gmax) - 2u;
+  const uint64_t ntiles_max = (nchunks + kTile - 1) / kTile;
+  const size_t sz_cnt = std::max<size_t>(1, nchunks * nslot_max);
+  const size_t text_bytes = (n * width + 255) & ~size_t(255);
+  auto carve = [&](arena& A, std::vector<void*>& v) {
+    v.clear();
+    v.push_back(A.take<uint64_t>(std::max<size_t>(1, size_t(gmax - 1) * stride)));  // 0 temp
+    v.push_back(A.take<uint32_t>(sz_cnt));                       // 1 counts
+    v.push_back(A.take<uint64_t>(sz_cnt));                       // 2 tnz scratch
+    v.push_back(A.take<uint64_t>(std::max<size_t>(1, ntiles_max * nslot_max)));  // 3 tsum
+    v.push_back(A.take<uint64_t>(nslot_max));                    // 4 totals
+    v.push_back(A.take<uint64_t>(nslot_max));                    // 5 netot
+    v.push_back(A.take<uint64_t>(nslot_max));                    // 6 gstart
+    v.push_back(A.take<uint64_t>(nslot_max));                    // 7 nebase
+    v.push_back(A.take<uint64_t>(kMaxGroup + 1));                // 8 level_ne
+    v.push_back(A.take<uint64_t>(sz_cnt));                       // 9 tab_dst
+    v.push_back(A.take<uint64_t>(sz_cnt));                       // 10 tab_src
+    v.push_back(A.take<uint64_t>(std::max<size_t>(1, nchunks << gmax)));  // 11 dstpos
+    v.push_back(A.take<uint64_t>(64));                           // 12 zeros
+    v.push_back(A.take<unsigned long long>(1));                  // 13 sym_or
+    v.push_back(A.take<uint8_t>(P > 1 ? text_bytes : 1));        // 14 reorder buffer A
+    v.push_back(A.take<uint8_t>(P > 2 ? text_bytes : 1));        // 15 reorder buffer B
+  };
+  std::vector<void*> b;
+  {
+    arena sizing;
+    sizing.cap = ~size_t(0) >> 1;
+    carve(sizing, b);
+    ctx->scratch.reset();
+    ctx->scratch.ensure(sizing.used);
+  }
+  carve(ctx->scratch, b);
+  auto U64 = [&](int i) { return static_cast<uint64_t*>(b[i]); };
+  uint32_t* counts = static_cast<uint32_t*>(b[1]);
+  PWM_HIP_TRY(hipMemsetAsync(b[13], 0, 8, s));
+  PWM_HIP_TRY(hipMemsetAsync(b[12], 0, 64 * 8, s));
+  const uint8_t* lut = compress_table(ctx);
+
+  PWM_HIP_TRY(hipEventRecord(ctx->ev[0], s));
+  const void* cur = d_text;
+  uint32_t l0 = 0;
+  for (uint32_t i = 0; i < P; ++i) {
+    const uint32_t g = gs[i];
+    const bool last = i + 1 == P;
+    const uint32_t nslot = last ? (1u << g) - 2u : (2u << g) - 2u;
+    K1Params k1{};
+    k1.text = cur;
+    k1.width = width;
+    k1.n = n;
+    k1.L = L;
+    k1.l0 = l0;
+    k1.g = g;
+    k1.np = g;
+    k1.out_level0 = ws->d_words + ws->level_offset[l0];
+    k1.temp_levels = U64(0);
+    k1.temp_payload = U64(0);
+    k1.temp_stride = stride;
+    k1.counts = counts;
+    k1.nslot = nslot;
+    k1.zeros_out = reinterpret_cast<unsigned long long*>(U64(12) + l0);
+    k1.sym_or = static_cast<unsigned long long*>(b[13]);
+    k1.nchunks = nchunks;
+    k1.lut = lut;
+    dispatch_small(ctx, k1, !last);
+    if (i == 0) PWM_HIP_TRY(hipEventRecord(ctx->ev[1], s));
+    if (nslot) {
+      const uint64_t ntiles = (nchunks + kTile - 1) / kTile;
+      hipLaunchKernelGGL(k_tile_sums2, dim3(uint32_t(ntiles), nslot), dim3(256), 0, s, counts, U64(3), U64(2),
+                         nchunks, nslot, ntiles);
+      hipLaunchKernelGGL(k_tile_scan, dim3(nslot), dim3(1024), 0, s, U64(3), U64(4), ntiles);
+      hipLaunchKernelGGL(k_tile_scan, dim3(nslot), dim3(1024), 0, s, U64(2), U64(5), ntiles);
+      const uint32_t jb = last ? g - 1 : g;  // sub-levels with bucket starts
+      hipLaunchKernelGGL(k_bucket_starts, dim3(jb), dim3(256), 0, s, U64(4), U64(6), 0, (uint64_t*)nullptr);
+      hipLaunchKernelGGL(k_bucket_starts, dim3(jb), dim3(256), 0, s, U64(5), U64(7), 0, U64(8));
+      hipLaunchKernelGGL(k_apply_runs, dim3(uint32_t(ntiles), nslot), dim3(1024), 0, s, counts, U64(3), U64(2),
+                         U64(6), U64(7), U64(9), U64(10), nchunks, nslot, ntiles, U64(11),
+                         last ? nslot : (1u << g) - 2u);
+      if (i == 0) PWM_HIP_TRY(hipEventRecord(ctx->ev[2], s));
+      const uint64_t W = words_for(n);
+      const uint32_t nwin = gather_windows(W, ctx->num_cus);
+      const uint32_t gblocks = uint32_t((W + 256ull * nwin - 1) / (256ull * nwin));
+      for (uint32_t j = 1; j < g; ++j) {
+        GatherParams gp{};
+        gp.nwin = nwin;
+        const uint64_t toff = uint64_t((1u << j) - 2) * nchunks;
+        gp.tab_dst = U64(9) + toff;
+        gp.tab_src = U64(10) + toff;
+        gp.level_ne = U64(8);
+        gp.j = j;
+        gp.n = n;
+        gp.src = U64(0) + (j - 1) * stride;
+        gp.dst = ws->d_words + ws->level_offset[l0 + j];
+        gp.nplanes = 1;
+        hipLaunchKernelGGL(k_gather, dim3(gblocks), dim3(256), 0, s, gp);
+      }
+      if (!last) {
+        ScatterParams sp{};
+        sp.in = cur;
+        sp.out = b[14 + (i & 1)];
+        sp.n = n;
+        sp.sh = L - l0 - g;
+        sp.dstpos = U64(11);
+        sp.nchunks = nchunks;
+        launch_scatter(s, width, g, sp);
+        cur = sp.out;
+      }
+    } else if (i == 0) {
+      PWM_HIP_TRY(hipEventRecord(ctx->ev[2], s));
+    }
+    PWM_HIP_TRY(hipGetLastError());
+    l0 += g;
+  }
+  PWM_HIP_TRY(hipEventRecord(ctx->ev[3], s));
+  if (ctx->pinned_words < 65) {
+    if (ctx->pinned) (void)hipHostFree(ctx->pinned);
+    void* hp = nullptr;
+    PWM_HIP_TRY(hipHostMalloc(&hp, 128 * 8, hipHostMallocDefault));
+    ctx->pinned = static_cast<uint64_t*>(hp);
+    ctx->pinned_words = 128;
+  }
+  PWM_HIP_TRY(hipMemcpyAsync(ctx->pinned, b[12], L * 8, hipMemcpyDeviceToHost, s));
+  PWM_HIP_TRY(hipMemcpyAsync(ctx->pinned + 64, b[13], 8, hipMemcpyDeviceToHost, s));
+  PWM_HIP_TRY(hipStreamSynchronize(s));
+  float ms = 0;
+  for (int k = 0; k < 3; ++k) {
+    PWM_HIP_TRY(hipEventElapsedTime(&ms, ctx->ev[k], ctx->ev[k + 1]));
+    ctx->last_ms[k] = ms;
+  }
+  PWM_HIP_TRY(hipEventElapsedTime(&ms, ctx->ev[0], ctx->ev[3]));
+  ctx->last_ms[3] = ms;
+  if (check_range && L < 64 && (ctx->pinned[64] >> L) != 0)
+    throw status_error{PWMHIP_ERR_SYMBOL_RANGE, "symbol >= 2^levels"};
+  uint64_t zeros[64];
+  std::memcpy(zeros, ctx->pinned, L * 8);
+  if (tree) wm_to_wt_device(ctx, d_text, n, width, L, ws);
+  ws->n_zeros = tree ? 0 : L;
+  for (uint32_t l = 0; l < L; ++l) ws->zeros[l] = tree ? 0 : zeros[l];
+}
+
 void build_plain_device(pwmhip_ctx* ctx, const void* d_text, uint64_t n, uint32_t width,
                         uint32_t L, bool tree, bool check_range, pwmhip_ws* ws) {
+  if (L > 4 && !small_k1_disabled()) {
+    build_passes_device(ctx, d_text, n, width, L, tree, check_range, ws);
+    return;
+  }
   hipStream_t s = ctx->stream;
   const uint64_t nchunks = (n + kChunkSyms - 1) / kChunkSyms;
   const uint64_t stride = level_stride(n);
@@ -1620,12 +1956,14 @@ void build_plain_device(pwmhip_ctx* ctx, const void* d_text, uint64_t n, uint32_
       hipLaunchKernelGGL(k_bucket_starts, dim3(jmax), dim3(256), 0, s, netot, nebase, tree_direct ? 1 : 0,
                          level_ne);
       hipLaunchKernelGGL(k_apply_runs, dim3(uint32_t(ntiles), nslot), dim3(1024), 0, s, counts, tsum, tnz, gstart,
-                         nebase, tab_dst, tab_src, nchunks, nslot, ntiles);
+                         nebase, tab_dst, tab_src, nchunks, nslot, ntiles, (uint64_t*)nullptr, nslot);
       if (gi == 0) PWM_HIP_TRY(hipEventRecord(ctx->ev[2], s));
       const uint64_t W = words_for(n);
-      const uint32_t gblocks = uint32_t((W + 256 * kGatherWindows - 1) / (256 * kGatherWindows));
+      const uint32_t nwin = gather_windows(W, ctx->num_cus);
+      const uint32_t gblocks = uint32_t((W + 256ull * nwin - 1) / (256ull * nwin));
       for (uint32_t j = 1; j <= jmax; ++j) {
         GatherParams gp{};
+        gp.nwin = nwin;
         const uint64_t toff = uint64_t((1u << j) - 2) * nchunks;
         gp.tab_dst = tab_dst + toff;
         gp.tab_src = tab_src + toff;
