@@ -38,6 +38,7 @@
 #include "pwm_internal.hpp"
 
 #include <cstdlib>
+#include <type_traits>
 #include <cstring>
 #include <algorithm>
 
@@ -847,37 +848,53 @@ __global__ __launch_bounds__(256) void k_scatter(ScatterParams p) {
   constexpr int WAVES = 4;
   constexpr int NQ = 1 << G;
   constexpr int CS = NQ + 1;  // per-lane counter stride (no bank conflicts)
-  __shared__ uint16_t stage[WAVES][kChunkSyms];  // chunk index of each sorted position
-  __shared__ uint8_t kbuf[WAVES][kChunkSyms];     // bucket of each symbol
+  // narrow symbols are staged by value, wider ones by chunk index
+  constexpr bool BYVAL = sizeof(T) <= 2;
+  using S = typename std::conditional<BYVAL, T, uint16_t>::type;
+  __shared__ S stage[WAVES][kChunkSyms];
+  __shared__ uint8_t kbuf[WAVES][kChunkSyms];  // key of symbol 64 * lane + i at [i * 64 + lane]
   __shared__ uint32_t cnt[WAVES][64 * CS];
   __shared__ uint64_t offs[WAVES][NQ];
   const int lane = threadIdx.x & 63, wid = threadIdx.x >> 6;
   const uint64_t c = uint64_t(blockIdx.x) * WAVES + wid;
   if (c >= p.nchunks) return;  // whole waves; no workgroup barrier below
-  const auto in = gmem(static_cast<const T*>(p.in) + c * kChunkSyms);
+  const T* chunk = static_cast<const T*>(p.in) + c * kChunkSyms;
   const uint32_t E = uint32_t(min<uint64_t>(kChunkSyms, p.n - c * kChunkSyms));
   const int m = max(0, min(64, int(E) - 64 * lane));
   auto key = [&](uint64_t x) -> uint32_t {
     const uint32_t f = uint32_t(x >> p.sh) & uint32_t(NQ - 1);
     return __builtin_bitreverse32(f) >> (32 - G);
   };
-  // keys of the lane's symbols into LDS (loads issued 16 at a time)
-  uint8_t* keys = &kbuf[wid][64 * lane];
-  const T* mine = static_cast<const T*>(p.in) + c * kChunkSyms + 64 * lane;
+  uint8_t* keys = &kbuf[wid][0];
+  // the lane's 64 symbols: 16-byte loads when the chunk is full and aligned
+  const bool vec = E == kChunkSyms && (reinterpret_cast<uintptr_t>(chunk) & 15) == 0;
+  constexpr int PER = 16 / sizeof(T);  // symbols per 16-byte load
+  if (vec) {
+    const auto src = gmem(reinterpret_cast<const uint4*>(chunk + 64 * lane));
 #pragma unroll
-  for (int blk = 0; blk < 4; ++blk) {
-    T v[16];
+    for (int blk = 0; blk < 64 / (4 * PER) + (64 % (4 * PER) ? 1 : 0); ++blk) {
+      uint4 v[4];
 #pragma unroll
-    for (int e = 0; e < 16; ++e) v[e] = (16 * blk + e < m) ? gmem(mine)[16 * blk + e] : T(0);
+      for (int u = 0; u < 4; ++u)
+        if ((blk * 4 + u) * PER < 64) v[u] = src[blk * 4 + u];
 #pragma unroll
-    for (int e = 0; e < 16; ++e) keys[16 * blk + e] = uint8_t(key(uint64_t(v[e])));
+      for (int u = 0; u < 4; ++u) {
+        if ((blk * 4 + u) * PER >= 64) continue;
+        const T* t = reinterpret_cast<const T*>(&v[u]);
+#pragma unroll
+        for (int e = 0; e < PER; ++e) keys[((blk * 4 + u) * PER + e) * 64 + lane] = uint8_t(key(uint64_t(t[e])));
+      }
+    }
+  } else {
+    const auto src = gmem(chunk + 64 * lane);
+    for (int i = 0; i < m; ++i) keys[i * 64 + lane] = uint8_t(key(uint64_t(src[i])));
   }
   uint32_t* my = &cnt[wid][lane * CS];
 #pragma unroll
   for (int q = 0; q < NQ; ++q) my[q] = 0;
-  for (int i = 0; i < m; ++i) ++my[keys[i]];
-  // lane prefixes and chunk totals per bucket (two 16-bit fields per scan)
-  // (each pair of counters is read before it is replaced by its start)
+  for (int i = 0; i < m; ++i) ++my[keys[i * 64 + lane]];
+  // lane prefixes and chunk totals per bucket (two 16-bit fields per scan;
+  // each pair of counters is read before it is replaced by its start)
   uint32_t run = 0;
 #pragma unroll
   for (int q = 0; q < NQ; q += 2) {
@@ -894,11 +911,13 @@ __global__ __launch_bounds__(256) void k_scatter(ScatterParams p) {
       run += t >> 16;
     }
   }
+  const auto in = gmem(chunk);
   for (int i = 0; i < m; ++i) {
-    const uint32_t q = keys[i];
+    const uint32_t q = keys[i * 64 + lane];
     const uint32_t pos = my[q];
     my[q] = pos + 1;
-    stage[wid][pos] = uint16_t(64 * lane + i);
+    if constexpr (BYVAL) stage[wid][pos] = in[64 * lane + i];
+    else stage[wid][pos] = uint16_t(64 * lane + i);
   }
   wave_sync();
   const auto out = gmem(static_cast<T*>(p.out));
@@ -908,7 +927,8 @@ __global__ __launch_bounds__(256) void k_scatter(ScatterParams p) {
 #pragma unroll
     for (int u = 0; u < 4; ++u) {
       const uint32_t k = k0 + 64 * u + lane;
-      x[u] = k < E ? in[stage[wid][k]] : T(0);
+      if constexpr (BYVAL) x[u] = k < E ? stage[wid][k] : T(0);
+      else x[u] = k < E ? in[stage[wid][k]] : T(0);
     }
 #pragma unroll
     for (int u = 0; u < 4; ++u) {
@@ -1203,7 +1223,49 @@ struct RunCache {
 };
 
 // One window (64 words from w0) whatever its number of runs.
-__device__ __forceinline__ void gather_window(const uint64_t* src, uint64_t* dst, RunCache& rc, uint64_t w0) {
+// A window that 63 or more runs overlap (sparse buckets: tiny runs from many
+// chunks) is assembled run by run: each lane takes one run, cuts it into
+// pieces of single destination words and ORs them into the wave's LDS copy
+// of the window; work is proportional to the runs, not to runs x words.
+__device__ void gather_window_runs(const uint64_t* src, uint64_t* dst, RunCache& rc, uint64_t w0, uint64_t e0,
+                                   unsigned long long* stage) {
+  const uint32_t lane = threadIdx.x & 63;
+  const uint64_t n = rc.n, NE = rc.NE;
+  const uint64_t sw0 = w0 << 6, sw1 = min<uint64_t>(sw0 + 4096, n);
+  // last run starting before sw1 (64-way ballot search from e0)
+  uint64_t lo = e0, hi = NE;
+  while (hi - lo > 1) {
+    const uint64_t step = (hi - lo + 63) >> 6;
+    const uint64_t cand = lo + uint64_t(lane + 1) * step;
+    const bool pred = cand < hi && rc.tab_dst[cand] < sw1;
+    lo += uint64_t(__popcll(__ballot(pred))) * step;
+    hi = min(hi, lo + step);
+  }
+  const uint64_t e1 = lo;
+  stage[lane] = 0;
+  wave_sync();
+  for (uint64_t eb = e0; eb <= e1; eb += 64) {
+    const uint64_t e = eb + lane;
+    if (e <= e1) {
+      const uint64_t d0 = rc.tab_dst[e], d1 = e + 1 < NE ? rc.tab_dst[e + 1] : n, s0 = rc.tab_src[e];
+      const uint64_t a = max(d0, sw0), b = min(d1, sw1);
+      for (uint64_t pos = a; pos < b;) {
+        const uint32_t off = uint32_t((pos - sw0) & 63);
+        const uint32_t len = uint32_t(min<uint64_t>(64 - off, b - pos));
+        const uint64_t v = (window64(src, int64_t(s0 + (pos - d0))) & range_mask(0, len)) >> off;
+        if (v) atomicOr(&stage[(pos - sw0) >> 6], (unsigned long long)v);
+        pos += len;
+      }
+    }
+  }
+  wave_sync();
+  if (((w0 + lane) << 6) < n) gmem(dst)[w0 + lane] = stage[lane];
+  rc.base = e1;  // the window's last run may continue into the next window
+  rc.load(lane);
+}
+
+__device__ __forceinline__ void gather_window(const uint64_t* src, uint64_t* dst, RunCache& rc, uint64_t w0,
+                                              unsigned long long* stage) {
   const uint32_t lane = threadIdx.x & 63;
   const uint64_t n = rc.n;
   const uint64_t sw0 = w0 << 6, sw1 = min<uint64_t>(sw0 + 4096, n);
@@ -1218,8 +1280,12 @@ __device__ __forceinline__ void gather_window(const uint64_t* src, uint64_t* dst
     ef = 0;
     el = __popcll(__ballot(rc.bd < sw1)) - 1;
   }
+  if (el == 63) {
+    gather_window_runs(src, dst, rc, w0, rc.base + ef, stage);
+    return;
+  }
   uint64_t v = 0;
-  if (el < 63 && el - ef < 4) {
+  if (el - ef < 4) {
     for (uint32_t e = ef; e <= el; ++e) {
       const uint64_t d0 = readlane64(rc.bd, e), d1 = readlane64(rc.bd, e + 1), s0 = readlane64(rc.bs, e);
       const uint64_t a = max(d0, ws), b = min(d1, we);
@@ -1229,13 +1295,8 @@ __device__ __forceinline__ void gather_window(const uint64_t* src, uint64_t* dst
       }
     }
   } else {
-    bool pending = live;
-    for (;;) {
-      v |= gather_word(src, rc.bd, rc.bs, ws, we, pending);
-      if (!__ballot(pending)) break;
-      rc.base += 63;  // the words continue past cached run 62
-      rc.load(lane);
-    }
+    bool pending = live;  // every run of the window is in the cache
+    v = gather_word(src, rc.bd, rc.bs, ws, we, pending);
   }
   if (live) gmem(dst)[w0 + lane] = v;
 }
@@ -1248,7 +1309,7 @@ __device__ __forceinline__ void gather_window(const uint64_t* src, uint64_t* dst
 // issues all their loads at once.  Other windows go one at a time.
 __device__ __forceinline__ void gather_range(const uint64_t* tab_dst, const uint64_t* tab_src, uint64_t NE, const uint64_t* src0,
                              uint64_t src_stride, uint64_t* dst0, uint64_t dst_stride, uint32_t nplanes, uint64_t n,
-                             uint64_t W0, uint32_t nwin) {
+                             uint64_t W0, uint32_t nwin, unsigned long long* stage) {
   const uint32_t lane = threadIdx.x & 63;
   const uint64_t X = W0 << 6;
   if (X >= n || NE == 0) return;
@@ -1329,7 +1390,7 @@ __device__ __forceinline__ void gather_range(const uint64_t* tab_dst, const uint
         for (uint32_t win = g0; win < min(g0 + kGroup, nwin); ++win) {
           const uint64_t w0 = W0 + 64ull * win;
           if ((w0 << 6) >= n) break;
-          gather_window(src, dst, rc, w0);
+          gather_window(src, dst, rc, w0, stage);
         }
       }
     }
@@ -1337,9 +1398,10 @@ __device__ __forceinline__ void gather_range(const uint64_t* tab_dst, const uint
 }
 
 __global__ __launch_bounds__(256) void k_gather(GatherParams p) {
+  __shared__ unsigned long long stage[4][64];
   const uint64_t W0 = (uint64_t(blockIdx.x) * 4 + (threadIdx.x >> 6)) * 64 * p.nwin;
   gather_range(p.tab_dst, p.tab_src, p.level_ne[p.j], p.src, p.src_stride, p.dst, p.dst_stride, p.nplanes, p.n,
-               W0, p.nwin);
+               W0, p.nwin, stage[threadIdx.x >> 6]);
 }
 
 // Batch: waves are assigned to texts by wave_base (per text), level j.
@@ -1360,6 +1422,7 @@ struct BatchGatherParams {
 };
 
 __global__ __launch_bounds__(256) void k_gather_batch(BatchGatherParams p) {
+  __shared__ unsigned long long stage[4][64];
   const uint64_t wv = uint64_t(blockIdx.x) * 4 + (threadIdx.x >> 6);
   if (wv >= p.wave_base[p.ntexts]) return;
   uint32_t lo = 0, hi = p.ntexts;  // wave_base[lo] <= wv < wave_base[hi]
@@ -1373,7 +1436,7 @@ __global__ __launch_bounds__(256) void k_gather_batch(BatchGatherParams p) {
   const uint64_t toff = uint64_t((1u << p.j) - 2) * p.nchunks + p.chunk_base[t] * (1ull << p.j);
   gather_range(p.tab_dst + toff, p.tab_src + toff, p.level_ne[uint64_t(t) * 8 + p.j],
                p.src + (p.j - 1) * p.src_stride, 0, p.out + p.out0[t] + p.j * p.stride[t], 0, 1, p.text_n[t], W0,
-               kGatherWindows);
+               kGatherWindows, stage[threadIdx.x >> 6]);
 }
 
 // Fused offsets for the run table: per (tile of 1024 chunks, slot) the sums of
