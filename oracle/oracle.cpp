@@ -430,4 +430,34 @@ void oracle_copy_bits8(uint8_t* dst, const uint8_t* src, uint64_t dst_off,
   }
 }
 
+// reduce_alphabet (util/alphabet_util.hpp:18-45): symbols renumbered in
+// first-occurrence order; returns levels_for_max_char(distinct - 1)
+// (util/alphabet_util.hpp:99-106).  The reference's byte table has 255
+// entries (byte 255 is undefined there); all 256 values are handled here.
+uint64_t oracle_reduce_alphabet(void* text, uint64_t n, uint32_t width) {
+  std::unordered_map<uint64_t, uint64_t> code;
+  for (uint64_t i = 0; i < n; ++i) {
+    const uint64_t c = load_sym(text, width, i);
+    if (!code.count(c)) {
+      const uint64_t k = code.size();
+      code.emplace(c, k);
+    }
+  }
+  for (uint64_t i = 0; i < n; ++i) {
+    const uint64_t v = code[load_sym(text, width, i)];
+    switch (width) {
+      case 1: static_cast<uint8_t*>(text)[i] = uint8_t(v); break;
+      case 2: static_cast<uint16_t*>(text)[i] = uint16_t(v); break;
+      case 4: static_cast<uint32_t*>(text)[i] = uint32_t(v); break;
+      default: static_cast<uint64_t*>(text)[i] = v; break;
+    }
+  }
+  uint64_t max_char = code.empty() ? 0 : code.size() - 1, levels = 0;
+  while (max_char) {
+    max_char >>= 1;
+    ++levels;
+  }
+  return levels;
+}
+
 }  // extern "C"

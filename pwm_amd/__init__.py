@@ -368,6 +368,28 @@ def build_batch(texts, levels, algo="ppc", is_tree=False, ctx=None, check_range=
     return res
 
 
+def reduce_alphabet(text, ctx=None):
+    """pwmhip_reduce_alphabet: renumbers the symbols of a device tensor in
+    place in first-occurrence order (reduce_alphabet,
+    util/alphabet_util.hpp:18-45) and returns the number of levels
+    (levels_for_max_char).  A numpy array is copied to the device, reduced
+    there and returned as a new array: (array, levels)."""
+    ctx = ctx or default_context()
+    levels = ctypes.c_uint64(0)
+    if hasattr(text, "data_ptr") and getattr(text, "is_cuda", False):
+        _check(_lib.pwmhip_reduce_alphabet(ctx.h, ctypes.c_void_p(text.data_ptr()), text.numel(),
+                                           text.element_size(), ctypes.byref(levels)), "reduce_alphabet")
+        return int(levels.value)
+    import torch
+    arr = np.ascontiguousarray(text)
+    signed = {1: np.int8, 2: np.int16, 4: np.int32, 8: np.int64}[arr.itemsize]
+    d = torch.from_numpy(arr.view(signed).copy()).to("cuda:%d" % ctx.device)
+    _check(_lib.pwmhip_reduce_alphabet(ctx.h, ctypes.c_void_p(d.data_ptr()), d.numel(), d.element_size(),
+                                       ctypes.byref(levels)), "reduce_alphabet")
+    torch.cuda.synchronize()
+    return d.cpu().numpy().view(arr.dtype), int(levels.value)
+
+
 class ConstructionAlgorithm:
     """Mirror of construction_algorithm (algorithm.hpp:83-191) for one GPU
     algorithm; `name` follows the reference's registrations

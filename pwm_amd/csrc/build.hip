@@ -829,11 +829,14 @@ __global__ __launch_bounds__(64 * 16) void k_local_small(K1Params p) {
 }
 
 // ---- symbol reorder between passes ------------------------------------------
-// After a pass over levels [l0, l0+G), the next pass reads the symbols in the
-// order of level l0+G: stable by bucket q = (bit of level l0+G-1 ... bit of
-// level l0), q ascending.  One wave per chunk: per-lane bucket counts in LDS,
-// wave scans for lane prefixes, the chunk's symbols placed in bucket order in
-// LDS, then written out so that each bucket's run is a contiguous store.
+// After a pass over the top G levels of its symbols, the next pass reads the
+// symbols in the order of the level below: stable by bucket q = (bit of the
+// pass's last level ... bit of its first level), q ascending.  Only the bits
+// below the pass are kept, in the narrowest type that holds them.  One wave
+// per chunk, 64 symbols per step (one per lane, coalesced): the step's bucket
+// masks come from G ballots; a lane's destination is its bucket's running
+// position (held by lane q for bucket q) plus its rank among the step's lanes
+// of that bucket.  No LDS, one pass over the chunk.
 struct ScatterParams {
   const void* in;
   void* out;
@@ -843,97 +846,40 @@ struct ScatterParams {
   uint64_t nchunks;
 };
 
-template <typename T, int G>
+template <typename TI, typename TO, int G>
 __global__ __launch_bounds__(256) void k_scatter(ScatterParams p) {
-  constexpr int WAVES = 4;
   constexpr int NQ = 1 << G;
-  constexpr int CS = NQ + 1;  // per-lane counter stride (no bank conflicts)
-  // narrow symbols are staged by value, wider ones by chunk index
-  constexpr bool BYVAL = sizeof(T) <= 2;
-  using S = typename std::conditional<BYVAL, T, uint16_t>::type;
-  __shared__ S stage[WAVES][kChunkSyms];
-  __shared__ uint8_t kbuf[WAVES][kChunkSyms];  // key of symbol 64 * lane + i at [i * 64 + lane]
-  __shared__ uint32_t cnt[WAVES][64 * CS];
-  __shared__ uint64_t offs[WAVES][NQ];
-  const int lane = threadIdx.x & 63, wid = threadIdx.x >> 6;
-  const uint64_t c = uint64_t(blockIdx.x) * WAVES + wid;
-  if (c >= p.nchunks) return;  // whole waves; no workgroup barrier below
-  const T* chunk = static_cast<const T*>(p.in) + c * kChunkSyms;
+  const int lane = threadIdx.x & 63;
+  const uint64_t c = uint64_t(blockIdx.x) * 4 + (threadIdx.x >> 6);
+  if (c >= p.nchunks) return;  // whole waves; no barrier below
+  const auto in = gmem(static_cast<const TI*>(p.in) + c * kChunkSyms);
+  const auto out = gmem(static_cast<TO*>(p.out));
   const uint32_t E = uint32_t(min<uint64_t>(kChunkSyms, p.n - c * kChunkSyms));
-  const int m = max(0, min(64, int(E) - 64 * lane));
-  auto key = [&](uint64_t x) -> uint32_t {
-    const uint32_t f = uint32_t(x >> p.sh) & uint32_t(NQ - 1);
-    return __builtin_bitreverse32(f) >> (32 - G);
-  };
-  uint8_t* keys = &kbuf[wid][0];
-  // the lane's 64 symbols: 16-byte loads when the chunk is full and aligned
-  const bool vec = E == kChunkSyms && (reinterpret_cast<uintptr_t>(chunk) & 15) == 0;
-  constexpr int PER = 16 / sizeof(T);  // symbols per 16-byte load
-  if (vec) {
-    const auto src = gmem(reinterpret_cast<const uint4*>(chunk + 64 * lane));
+  const uint64_t low = p.sh >= 64 ? ~0ull : ((1ull << p.sh) - 1);  // bits kept
+  uint64_t run = lane < NQ ? p.dstpos[c * NQ + lane] : 0;           // lane q: bucket q
+  const uint64_t lt = lane ? (~0ull >> (64 - lane)) : 0ull;
+  for (uint32_t t0 = 0; t0 < E; t0 += 256) {
+    TI x[4];
 #pragma unroll
-    for (int blk = 0; blk < 64 / (4 * PER) + (64 % (4 * PER) ? 1 : 0); ++blk) {
-      uint4 v[4];
+    for (int u = 0; u < 4; ++u) {
+      const uint32_t i = t0 + 64 * u + lane;
+      x[u] = i < E ? in[i] : TI(0);
+    }
 #pragma unroll
-      for (int u = 0; u < 4; ++u)
-        if ((blk * 4 + u) * PER < 64) v[u] = src[blk * 4 + u];
+    for (int u = 0; u < 4; ++u) {
+      const bool valid = t0 + 64 * u + lane < E;
+      const uint32_t f = uint32_t(uint64_t(x[u]) >> p.sh) & uint32_t(NQ - 1);
+      const uint32_t q = __builtin_bitreverse32(f) >> (32 - G);
+      uint64_t mine = __ballot(valid), own = mine;
 #pragma unroll
-      for (int u = 0; u < 4; ++u) {
-        if ((blk * 4 + u) * PER >= 64) continue;
-        const T* t = reinterpret_cast<const T*>(&v[u]);
-#pragma unroll
-        for (int e = 0; e < PER; ++e) keys[((blk * 4 + u) * PER + e) * 64 + lane] = uint8_t(key(uint64_t(t[e])));
+      for (int b = 0; b < G; ++b) {
+        const uint64_t B = __ballot(valid && ((q >> b) & 1));
+        mine &= ((q >> b) & 1) ? B : ~B;
+        own &= ((lane >> b) & 1) ? B : ~B;
       }
-    }
-  } else {
-    const auto src = gmem(chunk + 64 * lane);
-    for (int i = 0; i < m; ++i) keys[i * 64 + lane] = uint8_t(key(uint64_t(src[i])));
-  }
-  uint32_t* my = &cnt[wid][lane * CS];
-#pragma unroll
-  for (int q = 0; q < NQ; ++q) my[q] = 0;
-  for (int i = 0; i < m; ++i) ++my[keys[i * 64 + lane]];
-  // lane prefixes and chunk totals per bucket (two 16-bit fields per scan;
-  // each pair of counters is read before it is replaced by its start)
-  uint32_t run = 0;
-#pragma unroll
-  for (int q = 0; q < NQ; q += 2) {
-    const uint32_t v = my[q] | (q + 1 < NQ ? my[q + 1] << 16 : 0u);
-    const uint32_t inc = wave_incl_scan(v);
-    const uint32_t t = __builtin_amdgcn_readlane(inc, 63);
-    const uint32_t e = inc - v;
-    my[q] = run + (e & 0xffffu);
-    if (lane == 0) offs[wid][q] = p.dstpos[c * NQ + q] - run;
-    run += t & 0xffffu;
-    if (q + 1 < NQ) {
-      my[q + 1] = run + (e >> 16);
-      if (lane == 0) offs[wid][q + 1] = p.dstpos[c * NQ + q + 1] - run;
-      run += t >> 16;
-    }
-  }
-  const auto in = gmem(chunk);
-  for (int i = 0; i < m; ++i) {
-    const uint32_t q = keys[i * 64 + lane];
-    const uint32_t pos = my[q];
-    my[q] = pos + 1;
-    if constexpr (BYVAL) stage[wid][pos] = in[64 * lane + i];
-    else stage[wid][pos] = uint16_t(64 * lane + i);
-  }
-  wave_sync();
-  const auto out = gmem(static_cast<T*>(p.out));
-  // each bucket's run is a contiguous store; four symbols in flight per lane
-  for (uint32_t k0 = 0; k0 < E; k0 += 256) {
-    T x[4];
-#pragma unroll
-    for (int u = 0; u < 4; ++u) {
-      const uint32_t k = k0 + 64 * u + lane;
-      if constexpr (BYVAL) x[u] = k < E ? stage[wid][k] : T(0);
-      else x[u] = k < E ? in[stage[wid][k]] : T(0);
-    }
-#pragma unroll
-    for (int u = 0; u < 4; ++u) {
-      const uint32_t k = k0 + 64 * u + lane;
-      if (k < E) out[offs[wid][key(uint64_t(x[u]))] + k] = x[u];
+      const uint64_t base = __shfl(run, int(q), 64);
+      if (valid) out[base + __popcll(mine & lt)] = TO(uint64_t(x[u]) & low);
+      run += __popcll(own);
     }
   }
 }
@@ -1728,26 +1674,39 @@ uint32_t gather_windows(uint64_t W, uint32_t num_cus) {
   return uint32_t((nwin + kGroup - 1) / kGroup * kGroup);
 }
 
-template <int G>
-void launch_scatter_g(hipStream_t s, uint32_t width, const ScatterParams& sp) {
+template <typename TI, int G>
+void launch_scatter_o(hipStream_t s, uint32_t wout, const ScatterParams& sp) {
   const dim3 grid(uint32_t((sp.nchunks + 3) / 4)), block(256);
-  switch (width) {
-    case 1: hipLaunchKernelGGL((k_scatter<uint8_t, G>), grid, block, 0, s, sp); break;
-    case 2: hipLaunchKernelGGL((k_scatter<uint16_t, G>), grid, block, 0, s, sp); break;
-    case 4: hipLaunchKernelGGL((k_scatter<uint32_t, G>), grid, block, 0, s, sp); break;
-    default: hipLaunchKernelGGL((k_scatter<uint64_t, G>), grid, block, 0, s, sp); break;
+  switch (wout) {
+    case 1: hipLaunchKernelGGL((k_scatter<TI, uint8_t, G>), grid, block, 0, s, sp); break;
+    case 2: hipLaunchKernelGGL((k_scatter<TI, uint16_t, G>), grid, block, 0, s, sp); break;
+    case 4: hipLaunchKernelGGL((k_scatter<TI, uint32_t, G>), grid, block, 0, s, sp); break;
+    default: hipLaunchKernelGGL((k_scatter<TI, uint64_t, G>), grid, block, 0, s, sp); break;
   }
 }
 
-void launch_scatter(hipStream_t s, uint32_t width, uint32_t g, const ScatterParams& sp) {
+template <int G>
+void launch_scatter_g(hipStream_t s, uint32_t win, uint32_t wout, const ScatterParams& sp) {
+  switch (win) {
+    case 1: launch_scatter_o<uint8_t, G>(s, wout, sp); break;
+    case 2: launch_scatter_o<uint16_t, G>(s, wout, sp); break;
+    case 4: launch_scatter_o<uint32_t, G>(s, wout, sp); break;
+    default: launch_scatter_o<uint64_t, G>(s, wout, sp); break;
+  }
+}
+
+void launch_scatter(hipStream_t s, uint32_t win, uint32_t wout, uint32_t g, const ScatterParams& sp) {
   switch (g) {
-    case 1: launch_scatter_g<1>(s, width, sp); break;
-    case 2: launch_scatter_g<2>(s, width, sp); break;
-    case 3: launch_scatter_g<3>(s, width, sp); break;
-    default: launch_scatter_g<4>(s, width, sp); break;
+    case 1: launch_scatter_g<1>(s, win, wout, sp); break;
+    case 2: launch_scatter_g<2>(s, win, wout, sp); break;
+    case 3: launch_scatter_g<3>(s, win, wout, sp); break;
+    default: launch_scatter_g<4>(s, win, wout, sp); break;
   }
   PWM_HIP_TRY(hipGetLastError());
 }
+
+// Bytes per symbol that hold `bits` bits.
+uint32_t width_for_bits(uint32_t bits) { return bits <= 8 ? 1 : bits <= 16 ? 2 : bits <= 32 ? 4 : 8; }
 
 }  // namespace
 
@@ -1804,17 +1763,19 @@ void build_passes_device(pwmhip_ctx* ctx, const void* d_text, uint64_t n, uint32
 
   PWM_HIP_TRY(hipEventRecord(ctx->ev[0], s));
   const void* cur = d_text;
+  uint32_t cw = width;  // bytes per symbol of the current pass's input
   uint32_t l0 = 0;
   for (uint32_t i = 0; i < P; ++i) {
     const uint32_t g = gs[i];
     const bool last = i + 1 == P;
     const uint32_t nslot = last ? (1u << g) - 2u : (2u << g) - 2u;
+    const uint32_t R = L - l0;  // significant bits of the current symbols
     K1Params k1{};
     k1.text = cur;
-    k1.width = width;
+    k1.width = cw;
     k1.n = n;
-    k1.L = L;
-    k1.l0 = l0;
+    k1.L = R;
+    k1.l0 = 0;
     k1.g = g;
     k1.np = g;
     k1.out_level0 = ws->d_words + ws->level_offset[l0];
@@ -1864,11 +1825,13 @@ void build_passes_device(pwmhip_ctx* ctx, const void* d_text, uint64_t n, uint32
         sp.in = cur;
         sp.out = b[14 + (i & 1)];
         sp.n = n;
-        sp.sh = L - l0 - g;
+        sp.sh = R - g;
         sp.dstpos = U64(11);
         sp.nchunks = nchunks;
-        launch_scatter(s, width, g, sp);
+        const uint32_t wout = width_for_bits(R - g);
+        launch_scatter(s, cw, wout, g, sp);
         cur = sp.out;
+        cw = wout;
       }
     } else if (i == 0) {
       PWM_HIP_TRY(hipEventRecord(ctx->ev[2], s));

@@ -43,6 +43,8 @@ def oracle():
                                               vp, vp, vp, vp]
         lib.oracle_copy_bits64.argtypes = [vp, vp, ctypes.c_uint64, ctypes.c_uint64, ctypes.c_uint64]
         lib.oracle_copy_bits8.argtypes = [vp, vp, ctypes.c_uint64, ctypes.c_uint64, ctypes.c_uint64]
+        lib.oracle_reduce_alphabet.argtypes = [vp, ctypes.c_uint64, ctypes.c_uint32]
+        lib.oracle_reduce_alphabet.restype = ctypes.c_uint64
         _oracle = lib
     return _oracle
 
@@ -180,3 +182,10 @@ def ref_build(algo, text, levels, is_tree):
         return Structure(lv, bits, zeros[:nz], codes), secs.value
     finally:
         lib.ref_free(h)
+
+
+def reduce_alphabet(text):
+    """(reduced copy, levels): first-occurrence renumbering (alphabet_util.hpp:18-45)."""
+    t = np.array(text, copy=True)
+    levels = oracle().oracle_reduce_alphabet(_ptr(t), t.size, t.itemsize)
+    return t, int(levels)

@@ -124,3 +124,19 @@ def test_registry_names_and_traits():
     ws = a.compute(t, t.size, 8)
     o = O.naive(t, 8, False)
     _assert_same(ws, o.levels, "registry")
+
+
+@pytest.mark.gpu
+@pytest.mark.parametrize("w,L", [(4, 20), (2, 12), (1, 8)])
+def test_zipf_sparse_buckets_vs_oracle(w, L):
+    """Heavy-tailed symbols: deep levels have sparse buckets, so destination
+    windows are covered by many one-chunk runs (the run-by-run window path)
+    and passes reorder long stretches of equal symbols."""
+    t = pwm_amd.generate("zipf", 0xC3 + L, 1 << 21, w, L)
+    for tree in (False, True):
+        o = O.naive(t, L, tree)
+        ws = pwm_amd.build(t, L, "pps", tree, check_range=True)
+        _assert_same(ws, o.levels, ("zipf", w, L, tree))
+        if not tree:
+            assert np.array_equal(ws.zeros(), o.zeros)
+        ws.free()

@@ -221,3 +221,24 @@ def test_oracle_against_compiled_reference_random():
     O.ref().ref_rank1(w.ctypes.data, w.size, pos.ctypes.data, 300, out.ctypes.data)
     lib = O.oracle()
     assert all(out[i] == lib.oracle_rank1(w.ctypes.data, int(pos[i])) for i in range(300))
+
+
+def test_reduce_alphabet_first_occurrence():
+    """Closed form: code = rank of the symbol's first occurrence
+    (np.unique's return_index ordered), levels = bit length of distinct-1."""
+    rng = np.random.default_rng(11)
+    for dt, hi in ((np.uint8, 256), (np.uint16, 40000), (np.uint32, 2**32), (np.uint64, 2**63)):
+        t = rng.integers(0, hi, 5000, dtype=np.uint64).astype(dt)
+        t[:3] = t[10]  # repeats at the start
+        r, levels = O.reduce_alphabet(t)
+        vals, first = np.unique(t, return_index=True)
+        order = np.argsort(first)
+        rank = np.empty(vals.size, np.int64)
+        rank[order] = np.arange(vals.size)
+        want = rank[np.searchsorted(vals, t)]
+        assert np.array_equal(r.astype(np.int64), want)
+        assert levels == int(vals.size - 1).bit_length()
+    r, levels = O.reduce_alphabet(np.array([255, 0, 255], np.uint8))  # byte 255 handled
+    assert list(r) == [0, 1, 0] and levels == 1
+    r, levels = O.reduce_alphabet(np.array([7, 7], np.uint8))
+    assert list(r) == [0, 0] and levels == 0
