@@ -163,23 +163,33 @@ def run_batch(args):
     code = pwm_amd.ALGOS[algo]
     outs = (pwm_amd._WS * T)()
 
-    def batched():
-        st = lib.pwmhip_build_batch(ctx.h, ptrs, csz, T, 1, levels, code, flags | pwm_amd.FLAG_REUSE_OUTPUT, outs)
+    def batched(extra):
+        st = lib.pwmhip_build_batch(ctx.h, ptrs, csz, T, 1, levels, code,
+                                    flags | pwm_amd.FLAG_REUSE_OUTPUT | extra, outs)
         if st != pwm_amd.OK:
             raise pwm_amd.PwmError(st, "build_batch")
 
-    for _ in range(args.warmup):
-        batched()
-    torch.cuda.synchronize()
-    dev = []
-    t0 = time.perf_counter()
-    for _ in range(args.steps):
-        batched()
-        dev.append(ctx.last_timing())
-    torch.cuda.synchronize()
-    batch_ms = (time.perf_counter() - t0) / args.steps * 1e3
+    def timed(extra):
+        dev = []
+        torch.cuda.synchronize()
+        t0 = time.perf_counter()
+        for _ in range(args.steps):
+            batched(extra)
+            dev.append(ctx.last_timing())
+        torch.cuda.synchronize()
+        return (time.perf_counter() - t0) / args.steps * 1e3, dev
+
+    # the launch variants are tuned on the first call; the captured graph is
+    # replayed from the second on
+    graph_flags = 0 if args.no_graph else pwm_amd.FLAG_GRAPH | pwm_amd.FLAG_AUTOTUNE
+    for _ in range(max(args.warmup, 1)):
+        batched(graph_flags)
+    batch_ms, dev = timed(graph_flags)
     dev_ms = float(np.mean([d[3] for d in dev]))
-    phases = [round(float(np.mean([d[i] for d in dev])), 4) for i in range(3)]
+    # phase times come from the same tuned launches run eagerly (a graph is
+    # timed as a whole)
+    eager_ms, edev = timed(0)
+    phases = [round(float(np.mean([d[i] for d in edev])), 4) for i in range(3)]
 
     # one plain build per text, each into its own reused output
     singles = (pwm_amd._WS * T)()
@@ -248,6 +258,9 @@ def run_batch(args):
                    "texts": T, "symbols": total, "levels": levels,
                    "algorithm": ("wt_" if tree else "wm_") + algo,
                    "device_ms_per_build": round(dev_ms, 4),
+                   "graph": not args.no_graph,
+                   "eager_ms_per_build": round(eager_ms, 4),
+                   "eager_device_ms": round(float(np.mean([d[3] for d in edev])), 4),
                    "device_ms_phases": {"local_build": phases[0], "offsets": phases[1], "stitch": phases[2]},
                    "per_text_ms": round(single_ms, 2) if single_ms else None,
                    "per_text_device_ms": round(single_dev_ms, 2) if single_dev_ms else None,
@@ -402,6 +415,7 @@ def main():
     ap.add_argument("--no-per-text", action="store_true",
                     help="many-texts configs: skip the one-build-per-text comparison")
     ap.add_argument("--cpu-sample-mib", type=int, default=256)
+    ap.add_argument("--no-graph", action="store_true", help="batches: eager launches, default variants")
     ap.add_argument("--copy-mib", type=int, default=0,
                     help="only time a plain device-to-device copy of this many MiB (calibrates the "
                          "HBM byte counters of a profiler run)")

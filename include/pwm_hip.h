@@ -73,6 +73,10 @@ enum pwmhip_algo {
 #define PWMHIP_FLAG_CHECK_RANGE 4u     /* verify all symbols < 2^levels        */
 #define PWMHIP_FLAG_REUSE_OUTPUT 8u    /* `out` holds an earlier result whose   */
                                        /* device memory is reused if it fits   */
+#define PWMHIP_FLAG_GRAPH 16u          /* batches: replay the captured launch   */
+                                       /* sequence when the geometry repeats   */
+#define PWMHIP_FLAG_AUTOTUNE 32u       /* batches: time the launch variants of  */
+                                       /* a new length/level bucket, keep best */
 
 typedef struct pwmhip_ctx pwmhip_ctx;
 

@@ -38,6 +38,8 @@ FLAG_TREE = 1
 FLAG_TEXT_ON_DEVICE = 2
 FLAG_CHECK_RANGE = 4
 FLAG_REUSE_OUTPUT = 8
+FLAG_GRAPH = 16
+FLAG_AUTOTUNE = 32
 
 # algorithm ids (pwm_hip.h enum pwmhip_algo)
 ALGOS = {
@@ -321,11 +323,17 @@ class _BatchBlock:
         self._ws = None
 
 
-def build_batch(texts, levels, algo="ppc", is_tree=False, ctx=None, check_range=False):
+def build_batch(texts, levels, algo="ppc", is_tree=False, ctx=None, check_range=False, graph=False,
+                autotune=False, reuse=None):
     """pwmhip_build_batch: construct one structure per text in one launch per
     phase.  `texts` is a list of numpy arrays (host) or of device tensors
     (all of one kind and symbol width).  Returns a list of WaveletStructure;
-    each is equal to build(text, ...) of its text."""
+    each is equal to build(text, ...) of its text.  `graph` replays the
+    captured launch sequence when a batch repeats the previous one's
+    geometry and addresses; `autotune` times the launch variants once per
+    (levels, mean length) bucket of the context.  `reuse` is the list an
+    earlier build_batch returned: its device block is taken over (and those
+    structures become invalid)."""
     ctx = ctx or default_context()
     T = len(texts)
     if T == 0:
@@ -343,7 +351,20 @@ def build_batch(texts, levels, algo="ppc", is_tree=False, ctx=None, check_range=
         flags |= FLAG_TREE
     if check_range:
         flags |= FLAG_CHECK_RANGE
+    if graph:
+        flags |= FLAG_GRAPH
+    if autotune:
+        flags |= FLAG_AUTOTUNE
     outs = (_WS * T)()
+    if reuse:
+        blk = getattr(reuse[0], "_block", None)
+        if blk is not None and blk._ws is not None:
+            ctypes.memmove(ctypes.byref(outs[0]), ctypes.byref(blk._ws), ctypes.sizeof(_WS))
+            blk._ws = None
+            flags |= FLAG_REUSE_OUTPUT
+        for r in reuse:
+            r._ws = None
+            r._block = None
     st = _lib.pwmhip_build_batch(ctx.h, ptrs, sizes, T, widths.pop(), levels, ALGOS[algo], flags, outs)
     del keep
     _check(st, "build_batch %s" % algo)

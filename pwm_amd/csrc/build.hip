@@ -186,11 +186,10 @@ struct K1Params {
   unsigned long long* b_zeros;
   uint32_t ntexts;
   unsigned int* work;        // dynamic chunk queue (atomic counter), or nullptr
+  uint32_t grab;             // chunks taken from the queue at a time
+  uint32_t* b_chunk_text;    // batch mode: text of every chunk (written here)
   const uint8_t* lut;        // 64 KiB compress table in device memory
 };
-
-// Chunks a wave takes from the work queue at a time.
-constexpr uint32_t kGrab = 16;
 
 // Where a chunk's symbols come from and where its level-l0 word goes.
 struct ChunkView {
@@ -199,6 +198,7 @@ struct ChunkView {
   uint64_t n;                // the text's length
   uint64_t* lvl0;            // level-l0 word of the chunk's first 64 symbols
   unsigned long long* zout;  // zero counters of the chunk's text
+  uint32_t t;                // the chunk's text (batch mode)
 };
 
 // `hint` is the text of the wave's previous chunk: chunks arrive in
@@ -212,6 +212,7 @@ __device__ __forceinline__ ChunkView chunk_view(const K1Params& p, uint64_t c, u
     v.n = p.n;
     v.lvl0 = p.out_level0 + c * kChunkWords;
     v.zout = p.zeros_out;
+    v.t = 0;
     return v;
   }
   uint32_t lo = hint;
@@ -238,6 +239,7 @@ __device__ __forceinline__ ChunkView chunk_view(const K1Params& p, uint64_t c, u
   v.n = p.b_n[lo];
   v.lvl0 = p.out_level0 + p.b_out0[lo] + lc * kChunkWords;
   v.zout = p.b_zeros + uint64_t(lo) * kMaxGroup;
+  v.t = lo;
   return v;
 }
 
@@ -412,8 +414,8 @@ __global__ __launch_bounds__(64 * waves_for<NP>()) void k_local_build(K1Params p
     uint32_t q = 0;
     if (lane == 0) q = atomicAdd(p.work, 1u);
     q = __builtin_amdgcn_readfirstlane(q);
-    const uint64_t c0 = uint64_t(q) * kGrab;
-    grab_end = min<uint64_t>(c0 + kGrab, p.nchunks);
+    const uint64_t c0 = uint64_t(q) * p.grab;
+    grab_end = min<uint64_t>(c0 + p.grab, p.nchunks);
     return c0;
   };
   uint32_t hint = 0;
@@ -455,6 +457,7 @@ __global__ __launch_bounds__(64 * waves_for<NP>()) void k_local_build(K1Params p
     const uint64_t cn = next_chunk(c, false);
     ChunkView nv;
     if (cn < p.nchunks) nv = chunk_view(p, cn, hint);
+    if (p.b_chunk_text && lane == 0) p.b_chunk_text[c] = cv.t;
     if (cv.zout != zdst || zchunks == (1u << 19)) {
       flush_zeros();
       zdst = cv.zout;
@@ -651,8 +654,8 @@ __global__ __launch_bounds__(64 * 16) void k_local_small(K1Params p) {
     uint32_t q = 0;
     if (lane == 0) q = atomicAdd(p.work, 1u);
     q = __builtin_amdgcn_readfirstlane(q);
-    const uint64_t c0 = uint64_t(q) * kGrab;
-    grab_end = min<uint64_t>(c0 + kGrab, p.nchunks);
+    const uint64_t c0 = uint64_t(q) * p.grab;
+    grab_end = min<uint64_t>(c0 + p.grab, p.nchunks);
     return c0;
   };
   uint32_t hint = 0;
@@ -692,6 +695,7 @@ __global__ __launch_bounds__(64 * 16) void k_local_small(K1Params p) {
     const uint64_t cn = next_chunk(c, false);
     ChunkView nv;
     if (cn < p.nchunks) nv = chunk_view(p, cn, hint);
+    if (p.b_chunk_text && lane == 0) p.b_chunk_text[c] = cv.t;
     if (cv.zout != zdst || zchunks == (1u << 19)) {
       flush_zeros();
       zdst = cv.zout;
@@ -1365,6 +1369,7 @@ struct BatchGatherParams {
   uint64_t* out;
   uint32_t ntexts, j;
   uint64_t nchunks;
+  uint32_t nwin;               // windows per wave (multiple of kGroup)
 };
 
 __global__ __launch_bounds__(256) void k_gather_batch(BatchGatherParams p) {
@@ -1378,11 +1383,11 @@ __global__ __launch_bounds__(256) void k_gather_batch(BatchGatherParams p) {
     else hi = mid;
   }
   const uint32_t t = lo;
-  const uint64_t W0 = (wv - p.wave_base[t]) * 64 * kGatherWindows;
+  const uint64_t W0 = (wv - p.wave_base[t]) * 64 * p.nwin;
   const uint64_t toff = uint64_t((1u << p.j) - 2) * p.nchunks + p.chunk_base[t] * (1ull << p.j);
   gather_range(p.tab_dst + toff, p.tab_src + toff, p.level_ne[uint64_t(t) * 8 + p.j],
                p.src + (p.j - 1) * p.src_stride, 0, p.out + p.out0[t] + p.j * p.stride[t], 0, 1, p.text_n[t], W0,
-               kGatherWindows, stage[threadIdx.x >> 6]);
+               p.nwin, stage[threadIdx.x >> 6]);
 }
 
 // Fused offsets for the run table: per (tile of 1024 chunks, slot) the sums of
@@ -1470,21 +1475,101 @@ __global__ __launch_bounds__(1024) void k_apply_runs(const uint32_t* counts, con
 }
 
 // ---- batch offsets --------------------------------------------------------
-// Per text and slot: totals of counts and of non-empty indicators, as
-// differences of the global exclusive prefixes at the text's chunk range.
-__global__ void k_text_totals(const uint64_t* pref, const uint64_t* nepref, const uint64_t* totals,
-                              const uint64_t* netot, const uint64_t* chunk_base, uint64_t* ttot, uint64_t* tnz,
-                              uint32_t ntexts, uint32_t nslot, uint64_t nchunks) {
+// In-tile inclusive scans of one slot's counts and non-empty indicators over
+// the tile's 1024 chunks (one thread per chunk), plus the tile's offsets:
+// the chunk's global exclusive prefixes (pref, nzpref) and its count x.
+__device__ __forceinline__ void tile_prefix(const uint32_t* counts, const uint64_t* tsum, const uint64_t* tnz,
+                                            uint64_t nchunks, uint32_t nslot, uint64_t ntiles, uint64_t* wsum,
+                                            uint64_t& pref, uint64_t& nzpref, uint32_t& x) {
+  const uint64_t tile = blockIdx.x;
+  const uint32_t slot = blockIdx.y;
+  const int t = threadIdx.x, lane = t & 63, wid = t >> 6;
+  const uint64_t c = tile * kTile + t;
+  x = c < nchunks ? counts[c * nslot + slot] : 0;
+  uint64_t iv = x, iz = x != 0;
+#pragma unroll
+  for (int d = 1; d < 64; d <<= 1) {
+    const uint64_t a = __shfl_up(iv, d, 64), b = __shfl_up(iz, d, 64);
+    if (lane >= d) {
+      iv += a;
+      iz += b;
+    }
+  }
+  if (lane == 63) {
+    wsum[wid] = iv;
+    wsum[16 + wid] = iz;
+  }
+  __syncthreads();
+  uint64_t ov = tsum[uint64_t(slot) * ntiles + tile], oz = tnz[uint64_t(slot) * ntiles + tile];
+#pragma unroll
+  for (int w = 0; w < 16; ++w)
+    if (w < wid) {
+      ov += wsum[w];
+      oz += wsum[16 + w];
+    }
+  pref = ov + iv - x;
+  nzpref = oz + iz - (x != 0);
+}
+
+// Batch: global prefixes at every text's first chunk and inclusive ones at
+// its last chunk.
+__global__ __launch_bounds__(1024) void k_batch_text_bounds(const uint32_t* counts, const uint64_t* tsum,
+                                                            const uint64_t* tnz, const uint32_t* chunk_text,
+                                                            const uint64_t* chunk_base, uint64_t* tb_v,
+                                                            uint64_t* tb_z, uint64_t* te_v, uint64_t* te_z,
+                                                            uint64_t nchunks, uint32_t nslot, uint64_t ntiles) {
+  __shared__ uint64_t wsum[32];
+  uint64_t pref, nzpref;
+  uint32_t x;
+  tile_prefix(counts, tsum, tnz, nchunks, nslot, ntiles, wsum, pref, nzpref, x);
+  const uint64_t c = uint64_t(blockIdx.x) * kTile + threadIdx.x;
+  if (c >= nchunks) return;
+  const uint32_t slot = blockIdx.y, t = chunk_text[c];
+  if (chunk_base[t] == c) {
+    tb_v[uint64_t(t) * nslot + slot] = pref;
+    tb_z[uint64_t(t) * nslot + slot] = nzpref;
+  }
+  if (chunk_base[t + 1] == c + 1) {
+    te_v[uint64_t(t) * nslot + slot] = pref + x;
+    te_z[uint64_t(t) * nslot + slot] = nzpref + (x != 0);
+  }
+}
+
+__global__ void k_batch_text_totals(const uint64_t* tb_v, const uint64_t* tb_z, const uint64_t* te_v,
+                                    const uint64_t* te_z, const uint64_t* chunk_base, uint64_t* ttot,
+                                    uint64_t* tnz, uint32_t ntexts, uint32_t nslot) {
   const uint64_t i = uint64_t(blockIdx.x) * blockDim.x + threadIdx.x;
   if (i >= uint64_t(ntexts) * nslot) return;
-  const uint32_t t = uint32_t(i / nslot), slot = uint32_t(i % nslot);
-  const uint64_t c0 = chunk_base[t], c1 = chunk_base[t + 1];
-  const uint64_t p0 = c0 < nchunks ? pref[c0 * nslot + slot] : totals[slot];
-  const uint64_t p1 = c1 < nchunks ? pref[c1 * nslot + slot] : totals[slot];
-  const uint64_t z0 = c0 < nchunks ? nepref[c0 * nslot + slot] : netot[slot];
-  const uint64_t z1 = c1 < nchunks ? nepref[c1 * nslot + slot] : netot[slot];
-  ttot[i] = p1 - p0;
-  tnz[i] = z1 - z0;
+  const uint32_t t = uint32_t(i / nslot);
+  const bool empty = chunk_base[t] == chunk_base[t + 1];
+  ttot[i] = empty ? 0 : te_v[i] - tb_v[i];
+  tnz[i] = empty ? 0 : te_z[i] - tb_z[i];
+}
+
+// Batch run tables: every non-empty (chunk, bucket) run, text-local prefix =
+// global prefix - the text's base.
+__global__ __launch_bounds__(1024) void k_batch_apply_runs(const uint32_t* counts, const uint64_t* tsum,
+                                                           const uint64_t* tnz, const uint32_t* chunk_text,
+                                                           const uint64_t* chunk_base, const uint64_t* tb_v,
+                                                           const uint64_t* tb_z, const uint64_t* gstart,
+                                                           const uint64_t* nebase, uint64_t* tab_dst,
+                                                           uint64_t* tab_src, uint64_t nchunks, uint32_t nslot,
+                                                           uint64_t ntiles) {
+  __shared__ uint64_t wsum[32];
+  uint64_t pref, nzpref;
+  uint32_t x;
+  tile_prefix(counts, tsum, tnz, nchunks, nslot, ntiles, wsum, pref, nzpref, x);
+  const uint64_t c = uint64_t(blockIdx.x) * kTile + threadIdx.x;
+  if (c >= nchunks || !x) return;
+  const uint32_t slot = blockIdx.y, t = chunk_text[c];
+  const uint64_t ts = uint64_t(t) * nslot + slot;
+  const uint32_t j = slot_level(slot);
+  const uint32_t s0 = (1u << j) - 2;
+  uint64_t loc = 0;
+  for (uint32_t b = s0; b < slot; ++b) loc += counts[c * nslot + b];
+  const uint64_t e = uint64_t(s0) * nchunks + chunk_base[t] * (1ull << j) + nebase[ts] + (nzpref - tb_z[ts]);
+  tab_dst[e] = gstart[ts] + (pref - tb_v[ts]);
+  tab_src[e] = c * kChunkSyms + loc;
 }
 
 // Bucket starts of every (text, sub-level); grid (ntexts, jmax), 64 threads.
@@ -1507,33 +1592,9 @@ __global__ __launch_bounds__(64) void k_bucket_starts_batch(const uint64_t* ttot
   level_ne[uint64_t(t) * 8 + j] = runz;
 }
 
-__global__ void k_apply_runs_batch(const uint32_t* counts, const uint64_t* pref, const uint64_t* nepref,
-                                   const uint64_t* chunk_base, const uint64_t* gstart, const uint64_t* nebase,
-                                   uint64_t* tab_dst, uint64_t* tab_src, uint32_t ntexts, uint32_t nslot,
-                                   uint64_t nchunks) {
+__global__ void k_fill_u64(uint64_t* p, uint64_t n, uint64_t v) {
   const uint64_t i = uint64_t(blockIdx.x) * blockDim.x + threadIdx.x;
-  if (i >= nchunks * nslot) return;
-  const uint32_t x = counts[i];
-  if (!x) return;
-  const uint64_t c = i / nslot;
-  const uint32_t slot = uint32_t(i - c * nslot);
-  uint32_t lo = 0, hi = ntexts;
-  while (hi - lo > 1) {
-    const uint32_t mid = (lo + hi) >> 1;
-    if (chunk_base[mid] <= c) lo = mid;
-    else hi = mid;
-  }
-  const uint32_t t = lo;
-  const uint64_t c0 = chunk_base[t];
-  const uint64_t pin = pref[i] - pref[c0 * nslot + slot];
-  const uint64_t zin = nepref[i] - nepref[c0 * nslot + slot];
-  const uint32_t j = slot_level(slot);
-  const uint32_t s0 = (1u << j) - 2;
-  uint64_t loc = 0;
-  for (uint32_t b = s0; b < slot; ++b) loc += counts[c * nslot + b];
-  const uint64_t e = uint64_t(s0) * nchunks + c0 * (1ull << j) + nebase[uint64_t(t) * nslot + slot] + zin;
-  tab_dst[e] = gstart[uint64_t(t) * nslot + slot] + pin;
-  tab_src[e] = c * kChunkSyms + loc;
+  if (i < n) p[i] = v;
 }
 
 __global__ void k_fill_u32(unsigned int* p, uint32_t n, uint32_t v) {
@@ -2050,28 +2111,35 @@ void build_plain_device(pwmhip_ctx* ctx, const void* d_text, uint64_t n, uint32_
 // ---------------------------------------------------------------------------
 // batched construction: many texts, one launch per phase
 // ---------------------------------------------------------------------------
+namespace {
+
+struct BatchTune {
+  uint32_t grab;  // chunks per work-queue grab
+  uint32_t nwin;  // gather windows per wave
+};
+
+const BatchTune kTuneCandidates[] = {{4, 8}, {16, 8}, {64, 8}, {4, 32}, {16, 32}, {64, 32}};
+
+}  // namespace
+
 void build_batch_device(pwmhip_ctx* ctx, const void* const* d_texts, const uint64_t* sizes, uint64_t T,
                         uint32_t width, uint32_t L, bool tree, bool check_range, pwmhip_ws* outs, void* reuse,
-                        uint64_t reuse_bytes) {
+                        uint64_t reuse_bytes, uint32_t flags) {
   hipStream_t s = ctx->stream;
   const uint32_t g = L;  // one pass (L <= kMaxGroup)
   const uint32_t jmax = g - 1;
   const uint32_t nslot = jmax ? (2u << jmax) - 2u : 0;
   std::vector<uint64_t> cb(T + 1), out0(T), stride(T), wave_base(T + 1);
   std::vector<uint64_t> tptr(T), tn(T);
-  uint64_t words = 0, wv = 0;
-  const uint64_t per_wave = 64ull * kGatherWindows;
+  uint64_t words = 0;
   for (uint64_t t = 0; t < T; ++t) {
     cb[t + 1] = cb[t] + (sizes[t] + kChunkSyms - 1) / kChunkSyms;
     stride[t] = level_stride(sizes[t]);
     out0[t] = words;
     words += sizes[t] ? stride[t] * L : 0;
-    wave_base[t] = wv;
-    wv += (words_for(sizes[t]) + per_wave - 1) / per_wave;
     tptr[t] = reinterpret_cast<uint64_t>(d_texts[t]);
     tn[t] = sizes[t];
   }
-  wave_base[T] = wv;
   const uint64_t C = cb[T];
   // one allocation for all structures, owned by the first
   const uint64_t block_bytes = std::max<uint64_t>(words, 32) * 8;
@@ -2113,6 +2181,18 @@ void build_batch_device(pwmhip_ctx* ctx, const void* const* d_texts, const uint6
     return;
   }
 
+  // gather waves per text for a number of windows per wave
+  auto waves_for_nwin = [&](uint32_t nwin) {
+    const uint64_t per_wave = 64ull * nwin;
+    uint64_t wv = 0;
+    for (uint64_t t = 0; t < T; ++t) {
+      wave_base[t] = wv;
+      wv += (words_for(sizes[t]) + per_wave - 1) / per_wave;
+    }
+    wave_base[T] = wv;
+    return wv;
+  };
+
   const uint64_t tstride = C * kChunkWords + 64;
   const uint64_t ntiles = (C + kTile - 1) / kTile;
   const size_t sz_cnt = std::max<size_t>(1, C * nslot);
@@ -2129,9 +2209,9 @@ void build_batch_device(pwmhip_ctx* ctx, const void* const* d_texts, const uint6
     v.push_back(A.take<uint64_t>(T * kMaxGroup));         // 6 zeros
     v.push_back(A.take<uint64_t>(std::max<size_t>(1, size_t(L - 1)) * tstride));  // 7 temp
     v.push_back(A.take<uint32_t>(sz_cnt));                // 8 counts
-    v.push_back(A.take<uint64_t>(sz_cnt));                // 9 pref
-    v.push_back(A.take<uint64_t>(sz_cnt));                // 10 nepref
-    v.push_back(A.take<uint64_t>(std::max<size_t>(1, ntiles * nslot)));  // 11 tiles
+    v.push_back(A.take<uint64_t>(std::max<size_t>(1, ntiles * nslot)));  // 9 tile sums
+    v.push_back(A.take<uint64_t>(std::max<size_t>(1, ntiles * nslot)));  // 10 tile indicator sums
+    v.push_back(A.take<uint64_t>(1));                     // 11 (unused)
     v.push_back(A.take<uint64_t>(std::max<uint32_t>(nslot, 1)));          // 12 totals
     v.push_back(A.take<uint64_t>(std::max<uint32_t>(nslot, 1)));          // 13 netot
     v.push_back(A.take<uint64_t>(std::max<size_t>(1, T * nslot)));        // 14 ttot
@@ -2143,6 +2223,9 @@ void build_batch_device(pwmhip_ctx* ctx, const void* const* d_texts, const uint6
     v.push_back(A.take<uint64_t>(sz_cnt));                // 20 tab_src
     v.push_back(A.take<unsigned long long>(1));           // 21 sym_or
     v.push_back(A.take<unsigned int>(1));                 // 22 work counter
+    v.push_back(A.take<uint32_t>(std::max<size_t>(1, C))); // 23 chunk -> text
+    for (int k = 0; k < 4; ++k)                            // 24..27 text bounds
+      v.push_back(A.take<uint64_t>(std::max<size_t>(1, T * nslot)));
   };
   std::vector<void*> b;
   carve(sizing, b);
@@ -2150,86 +2233,197 @@ void build_batch_device(pwmhip_ctx* ctx, const void* const* d_texts, const uint6
   ctx->scratch.ensure(sizing.used);
   carve(ctx->scratch, b);
   auto U64 = [&](int i) { return static_cast<uint64_t*>(b[i]); };
-  PWM_HIP_TRY(hipMemcpyAsync(b[0], cb.data(), (T + 1) * 8, hipMemcpyHostToDevice, s));
-  PWM_HIP_TRY(hipMemcpyAsync(b[1], tptr.data(), T * 8, hipMemcpyHostToDevice, s));
-  PWM_HIP_TRY(hipMemcpyAsync(b[2], tn.data(), T * 8, hipMemcpyHostToDevice, s));
-  PWM_HIP_TRY(hipMemcpyAsync(b[3], out0.data(), T * 8, hipMemcpyHostToDevice, s));
-  PWM_HIP_TRY(hipMemcpyAsync(b[4], stride.data(), T * 8, hipMemcpyHostToDevice, s));
-  PWM_HIP_TRY(hipMemcpyAsync(b[5], wave_base.data(), (T + 1) * 8, hipMemcpyHostToDevice, s));
-  PWM_HIP_TRY(hipMemsetAsync(b[6], 0, T * kMaxGroup * 8, s));
-  PWM_HIP_TRY(hipMemsetAsync(b[21], 0, 8, s));
-  hipLaunchKernelGGL(k_fill_u32, dim3(1), dim3(64), 0, s, static_cast<unsigned int*>(b[22]), 1u, 0u);
+  const uint8_t* lut = compress_table(ctx);  // before any capture
 
-  PWM_HIP_TRY(hipEventRecord(ctx->ev[0], s));
-  K1Params k1{};
-  k1.width = width;
-  k1.n = 0;
-  k1.L = L;
-  k1.l0 = 0;
-  k1.g = g;
-  k1.np = L;
-  k1.out_level0 = static_cast<uint64_t*>(block);
-  k1.temp_levels = U64(7);
-  k1.temp_payload = U64(7);
-  k1.temp_stride = tstride;
-  k1.counts = static_cast<uint32_t*>(b[8]);
-  k1.nslot = nslot;
-  k1.sym_or = static_cast<unsigned long long*>(b[21]);
-  k1.nchunks = C;
-  k1.lut = compress_table(ctx);
-  k1.b_chunk_base = U64(0);
-  k1.b_text = U64(1);
-  k1.b_n = U64(2);
-  k1.b_out0 = U64(3);
-  k1.b_zeros = static_cast<unsigned long long*>(b[6]);
-  k1.ntexts = uint32_t(T);
-  k1.work = static_cast<unsigned int*>(b[22]);
-  dispatch_k1(ctx, k1);
-  PWM_HIP_TRY(hipEventRecord(ctx->ev[1], s));
-  if (nslot) {
-    const uint32_t* counts = static_cast<const uint32_t*>(b[8]);
-    launch_scan_slots(s, counts, U64(9), U64(12), C, nslot, U64(11), false);
-    launch_scan_slots(s, counts, U64(10), U64(13), C, nslot, U64(11), true);
-    const uint64_t tsl = T * nslot;
-    hipLaunchKernelGGL(k_text_totals, dim3(uint32_t((tsl + 255) / 256)), dim3(256), 0, s, U64(9), U64(10), U64(12),
-                       U64(13), U64(0), U64(14), U64(15), uint32_t(T), nslot, C);
-    hipLaunchKernelGGL(k_bucket_starts_batch, dim3(uint32_t(T), jmax), dim3(64), 0, s, U64(14), U64(15), U64(16),
-                       U64(17), U64(18), nslot, tree ? 1 : 0);
-    const uint64_t nth = C * nslot;
-    hipLaunchKernelGGL(k_apply_runs_batch, dim3(uint32_t((nth + 255) / 256)), dim3(256), 0, s, counts, U64(9),
-                       U64(10), U64(0), U64(16), U64(17), U64(19), U64(20), uint32_t(T), nslot, C);
-    PWM_HIP_TRY(hipEventRecord(ctx->ev[2], s));
-    for (uint32_t j = 1; j <= jmax; ++j) {
-      BatchGatherParams bp{};
-      bp.wave_base = U64(5);
-      bp.chunk_base = U64(0);
-      bp.text_n = U64(2);
-      bp.out0 = U64(3);
-      bp.stride = U64(4);
-      bp.level_ne = U64(18);
-      bp.tab_dst = U64(19);
-      bp.tab_src = U64(20);
-      bp.src = U64(7);
-      bp.src_stride = tstride;
-      bp.out = static_cast<uint64_t*>(block);
-      bp.ntexts = uint32_t(T);
-      bp.j = j;
-      bp.nchunks = C;
-      hipLaunchKernelGGL(k_gather_batch, dim3(uint32_t((wv + 3) / 4)), dim3(256), 0, s, bp);
-    }
-  } else {
-    PWM_HIP_TRY(hipEventRecord(ctx->ev[2], s));
+  // the six metadata arrays are consecutive in the arena: one pinned copy
+  // with the same layout (zero counts come back through the same buffer)
+  const size_t span = size_t(static_cast<char*>(b[5]) - static_cast<char*>(b[0])) + (T + 1) * 8;
+  const size_t zbytes = T * kMaxGroup * 8;
+  if (ctx->hstage_bytes < std::max(span, zbytes + 64)) {
+    if (ctx->hstage) (void)hipHostFree(ctx->hstage);
+    ctx->hstage = nullptr;
+    ctx->hstage_bytes = 0;
+    void* hp = nullptr;
+    const size_t want = std::max(span, zbytes + 64) * 2;
+    PWM_HIP_TRY(hipHostMalloc(&hp, want, hipHostMallocDefault));
+    ctx->hstage = hp;
+    ctx->hstage_bytes = want;
   }
-  PWM_HIP_TRY(hipGetLastError());
-  PWM_HIP_TRY(hipEventRecord(ctx->ev[3], s));
-  std::vector<uint64_t> zeros(T * kMaxGroup);
-  uint64_t sym_or = 0;
-  PWM_HIP_TRY(hipMemcpyAsync(zeros.data(), b[6], T * kMaxGroup * 8, hipMemcpyDeviceToHost, s));
-  PWM_HIP_TRY(hipMemcpyAsync(&sym_or, b[21], 8, hipMemcpyDeviceToHost, s));
+  char* hs = static_cast<char*>(ctx->hstage);
+  auto put = [&](int k, const std::vector<uint64_t>& v, size_t cnt) {
+    std::memcpy(hs + (static_cast<char*>(b[k]) - static_cast<char*>(b[0])), v.data(), cnt * 8);
+  };
+  // metadata for a gather geometry; the previous use of the staging buffer
+  // has completed (every upload is followed by a synchronisation)
+  auto upload = [&](uint32_t nwin) {
+    const uint64_t wv = waves_for_nwin(nwin);
+    put(0, cb, T + 1);
+    put(1, tptr, T);
+    put(2, tn, T);
+    put(3, out0, T);
+    put(4, stride, T);
+    put(5, wave_base, T + 1);
+    PWM_HIP_TRY(hipMemcpyAsync(b[0], hs, span, hipMemcpyHostToDevice, s));
+    return wv;
+  };
+
+  // the device sequence: clears (kernels, so that it can be captured), local
+  // build, offsets, gathers
+  auto enqueue = [&](const BatchTune& tu, uint64_t wv, bool events) {
+    const uint64_t nz = T * kMaxGroup;
+    hipLaunchKernelGGL(k_fill_u64, dim3(uint32_t((nz + 255) / 256)), dim3(256), 0, s, U64(6), nz, uint64_t(0));
+    hipLaunchKernelGGL(k_fill_u64, dim3(1), dim3(64), 0, s, U64(21), uint64_t(1), uint64_t(0));
+    hipLaunchKernelGGL(k_fill_u32, dim3(1), dim3(64), 0, s, static_cast<unsigned int*>(b[22]), 1u, 0u);
+    if (events) PWM_HIP_TRY(hipEventRecord(ctx->ev[0], s));
+    K1Params k1{};
+    k1.width = width;
+    k1.n = 0;
+    k1.L = L;
+    k1.l0 = 0;
+    k1.g = g;
+    k1.np = L;
+    k1.out_level0 = static_cast<uint64_t*>(block);
+    k1.temp_levels = U64(7);
+    k1.temp_payload = U64(7);
+    k1.temp_stride = tstride;
+    k1.counts = static_cast<uint32_t*>(b[8]);
+    k1.nslot = nslot;
+    k1.sym_or = static_cast<unsigned long long*>(b[21]);
+    k1.nchunks = C;
+    k1.lut = lut;
+    k1.b_chunk_base = U64(0);
+    k1.b_text = U64(1);
+    k1.b_n = U64(2);
+    k1.b_out0 = U64(3);
+    k1.b_zeros = static_cast<unsigned long long*>(b[6]);
+    k1.ntexts = uint32_t(T);
+    k1.work = static_cast<unsigned int*>(b[22]);
+    k1.grab = tu.grab;
+    k1.b_chunk_text = static_cast<uint32_t*>(b[23]);
+    dispatch_k1(ctx, k1);
+    if (events) PWM_HIP_TRY(hipEventRecord(ctx->ev[1], s));
+    if (nslot) {
+      const uint32_t* counts = static_cast<const uint32_t*>(b[8]);
+      // tile sums, their scans, the text bounds, per-text bucket starts, then
+      // the run tables (counts read twice, no per-chunk prefix arrays)
+      const uint32_t* ctext = static_cast<const uint32_t*>(b[23]);
+      hipLaunchKernelGGL(k_tile_sums2, dim3(uint32_t(ntiles), nslot), dim3(256), 0, s, counts, U64(9), U64(10), C,
+                         nslot, ntiles);
+      hipLaunchKernelGGL(k_tile_scan, dim3(nslot), dim3(1024), 0, s, U64(9), U64(12), ntiles);
+      hipLaunchKernelGGL(k_tile_scan, dim3(nslot), dim3(1024), 0, s, U64(10), U64(13), ntiles);
+      hipLaunchKernelGGL(k_batch_text_bounds, dim3(uint32_t(ntiles), nslot), dim3(1024), 0, s, counts, U64(9),
+                         U64(10), ctext, U64(0), U64(24), U64(25), U64(26), U64(27), C, nslot, ntiles);
+      const uint64_t tsl = T * nslot;
+      hipLaunchKernelGGL(k_batch_text_totals, dim3(uint32_t((tsl + 255) / 256)), dim3(256), 0, s, U64(24),
+                         U64(25), U64(26), U64(27), U64(0), U64(14), U64(15), uint32_t(T), nslot);
+      hipLaunchKernelGGL(k_bucket_starts_batch, dim3(uint32_t(T), jmax), dim3(64), 0, s, U64(14), U64(15),
+                         U64(16), U64(17), U64(18), nslot, tree ? 1 : 0);
+      hipLaunchKernelGGL(k_batch_apply_runs, dim3(uint32_t(ntiles), nslot), dim3(1024), 0, s, counts, U64(9),
+                         U64(10), ctext, U64(0), U64(24), U64(25), U64(16), U64(17), U64(19), U64(20), C, nslot,
+                         ntiles);
+      if (events) PWM_HIP_TRY(hipEventRecord(ctx->ev[2], s));
+      for (uint32_t j = 1; j <= jmax; ++j) {
+        BatchGatherParams bp{};
+        bp.wave_base = U64(5);
+        bp.chunk_base = U64(0);
+        bp.text_n = U64(2);
+        bp.out0 = U64(3);
+        bp.stride = U64(4);
+        bp.level_ne = U64(18);
+        bp.tab_dst = U64(19);
+        bp.tab_src = U64(20);
+        bp.src = U64(7);
+        bp.src_stride = tstride;
+        bp.out = static_cast<uint64_t*>(block);
+        bp.ntexts = uint32_t(T);
+        bp.j = j;
+        bp.nchunks = C;
+        bp.nwin = tu.nwin;
+        hipLaunchKernelGGL(k_gather_batch, dim3(uint32_t((wv + 3) / 4)), dim3(256), 0, s, bp);
+      }
+    } else if (events) {
+      PWM_HIP_TRY(hipEventRecord(ctx->ev[2], s));
+    }
+    PWM_HIP_TRY(hipGetLastError());
+    if (events) PWM_HIP_TRY(hipEventRecord(ctx->ev[3], s));
+  };
+
+  // tunables of this length / level bucket (mean text length, rounded to a
+  // power of two, and the number of levels)
+  uint32_t lg = 0;
+  for (uint64_t m = std::max<uint64_t>(1, words_for(cb[T] * kChunkSyms) / T); m > 1; m >>= 1) ++lg;
+  const uint64_t bucket = (uint64_t(L) << 8) | lg;
+  BatchTune tu{16, kGatherWindows};
+  auto it = ctx->tune.find(bucket);
+  if (it != ctx->tune.end()) {
+    tu = BatchTune{it->second.first, it->second.second};
+  } else if (flags & PWMHIP_FLAG_AUTOTUNE) {
+    float best = 0;
+    for (const BatchTune& cand : kTuneCandidates) {
+      const uint64_t wv = upload(cand.nwin);
+      enqueue(cand, wv, true);
+      PWM_HIP_TRY(hipStreamSynchronize(s));
+      float ms = 0;
+      PWM_HIP_TRY(hipEventElapsedTime(&ms, ctx->ev[0], ctx->ev[3]));
+      if (best == 0 || ms < best) {
+        best = ms;
+        tu = cand;
+      }
+    }
+    ctx->tune[bucket] = {tu.grab, tu.nwin};
+  }
+
+  const uint64_t wv = upload(tu.nwin);
+  bool timed = true;
+  if (flags & PWMHIP_FLAG_GRAPH) {
+    // replay when the geometry and every address the kernels see repeat
+    const uint64_t key[12] = {T,
+                              C,
+                              wv,
+                              width,
+                              L,
+                              uint64_t(tree),
+                              reinterpret_cast<uint64_t>(block),
+                              reinterpret_cast<uint64_t>(ctx->scratch.base),
+                              sizing.used,
+                              tu.grab,
+                              tu.nwin,
+                              reinterpret_cast<uint64_t>(lut)};
+    if (!ctx->gexec || std::memcmp(key, ctx->gkey, sizeof(key)) != 0) {
+      if (ctx->gexec) (void)hipGraphExecDestroy(ctx->gexec);
+      if (ctx->graph) (void)hipGraphDestroy(ctx->graph);
+      ctx->gexec = nullptr;
+      ctx->graph = nullptr;
+      PWM_HIP_TRY(hipStreamBeginCapture(s, hipStreamCaptureModeThreadLocal));
+      try {
+        enqueue(tu, wv, false);
+      } catch (...) {
+        hipGraph_t broken = nullptr;
+        (void)hipStreamEndCapture(s, &broken);
+        if (broken) (void)hipGraphDestroy(broken);
+        throw;
+      }
+      PWM_HIP_TRY(hipStreamEndCapture(s, &ctx->graph));
+      PWM_HIP_TRY(hipGraphInstantiate(&ctx->gexec, ctx->graph, nullptr, nullptr, 0));
+      std::memcpy(ctx->gkey, key, sizeof(key));
+    }
+    PWM_HIP_TRY(hipEventRecord(ctx->ev[0], s));
+    PWM_HIP_TRY(hipGraphLaunch(ctx->gexec, s));
+    PWM_HIP_TRY(hipEventRecord(ctx->ev[3], s));
+    timed = false;  // phases are not separated inside the graph
+  } else {
+    enqueue(tu, wv, true);
+  }
+  // (the metadata upload above is ordered before these downloads on the stream)
+  uint64_t* zeros = reinterpret_cast<uint64_t*>(hs);
+  PWM_HIP_TRY(hipMemcpyAsync(zeros, b[6], zbytes, hipMemcpyDeviceToHost, s));
+  PWM_HIP_TRY(hipMemcpyAsync(zeros + T * kMaxGroup, b[21], 8, hipMemcpyDeviceToHost, s));
   PWM_HIP_TRY(hipStreamSynchronize(s));
+  const uint64_t sym_or = zeros[T * kMaxGroup];
   float ms = 0;
   for (int i = 0; i < 3; ++i) {
-    PWM_HIP_TRY(hipEventElapsedTime(&ms, ctx->ev[i], ctx->ev[i + 1]));
+    ms = 0;
+    if (timed) PWM_HIP_TRY(hipEventElapsedTime(&ms, ctx->ev[i], ctx->ev[i + 1]));
     ctx->last_ms[i] = ms;
   }
   PWM_HIP_TRY(hipEventElapsedTime(&ms, ctx->ev[0], ctx->ev[3]));

@@ -125,6 +125,9 @@ int pwmhip_destroy(pwmhip_ctx* ctx) {
   ctx->rs_scratch.release();
   if (ctx->pinned) (void)hipHostFree(ctx->pinned);
   if (ctx->d_lut) (void)hipFree(ctx->d_lut);
+  if (ctx->hstage) (void)hipHostFree(ctx->hstage);
+  if (ctx->gexec) (void)hipGraphExecDestroy(ctx->gexec);
+  if (ctx->graph) (void)hipGraphDestroy(ctx->graph);
   for (auto& e : ctx->ev)
     if (e) (void)hipEventDestroy(e);
   if (ctx->stream) (void)hipStreamDestroy(ctx->stream);
@@ -208,7 +211,20 @@ int pwmhip_build_batch(pwmhip_ctx* ctx, const void* const* texts, const uint64_t
   for (uint64_t t = 0; t < count; ++t) {
     std::free(outs[t].code_len);
     std::free(outs[t].code_word);
-    std::memset(&outs[t], 0, sizeof(pwmhip_ws));
+    // the fixed header fields only: level arrays are written up to `levels`
+    // by the build and zeros up to n_zeros (a full clear of 20,000 structures
+    // is tens of megabytes)
+    pwmhip_ws& w = outs[t];
+    w.levels = 0;
+    w.text_size = 0;
+    w.is_tree = 0;
+    w.is_huffman = 0;
+    w.d_words = nullptr;
+    w.n_zeros = 0;
+    w.n_codes = 0;
+    w.code_len = nullptr;
+    w.code_word = nullptr;
+    w.alloc_bytes = 0;
   }
   if (word_width != 1 && word_width != 2 && word_width != 4 && word_width != 8) return PWMHIP_ERR_INVALID;
   if (algo > PWMHIP_ALGO_HUFF_DD_PC) return PWMHIP_ERR_INVALID;
@@ -255,7 +271,7 @@ int pwmhip_build_batch(pwmhip_ctx* ctx, const void* const* texts, const uint64_t
       }
     }
     pwm::build_batch_device(ctx, d_texts.data(), sizes, count, word_width, levels, tree,
-                            (flags & PWMHIP_FLAG_CHECK_RANGE) != 0, outs, reuse_ptr, reuse_bytes);
+                            (flags & PWMHIP_FLAG_CHECK_RANGE) != 0, outs, reuse_ptr, reuse_bytes, flags);
     if (!tree && pwm::zeros_last_is_zero(algo))
       for (uint64_t t = 0; t < count; ++t)
         if (outs[t].levels) outs[t].zeros[levels - 1] = 0;

@@ -4,6 +4,8 @@
 #include <hip/hip_runtime.h>
 
 #include <cstdint>
+#include <unordered_map>
+#include <utility>
 #include <cstdio>
 #include <string>
 #include <vector>
@@ -81,6 +83,15 @@ struct pwmhip_ctx {
   size_t pinned_words = 0;
   // 64 KiB byte-compress table (lut[(m << 8) | x] = pext(x, m)), filled once
   uint8_t* d_lut = nullptr;
+  // pinned host staging of batched builds (metadata up, zero counts down)
+  void* hstage = nullptr;
+  size_t hstage_bytes = 0;
+  // batched builds: tuned (work-queue grab, gather windows) per length /
+  // level bucket, and the last captured launch sequence with its geometry
+  std::unordered_map<uint64_t, std::pair<uint32_t, uint32_t>> tune;
+  hipGraph_t graph = nullptr;
+  hipGraphExec_t gexec = nullptr;
+  uint64_t gkey[12] = {};
 };
 
 namespace pwm {
@@ -115,9 +126,10 @@ void build_plain_device(pwmhip_ctx* ctx, const void* d_text, uint64_t n, uint32_
 // others are views with alloc_bytes == 0).  levels <= kMaxGroup.  `reuse`
 // (reuse_bytes) is an earlier batch's block, taken over when large enough
 // and freed otherwise.
+// flags: PWMHIP_FLAG_GRAPH / PWMHIP_FLAG_AUTOTUNE.
 void build_batch_device(pwmhip_ctx* ctx, const void* const* d_texts, const uint64_t* sizes, uint64_t T,
                         uint32_t width, uint32_t L, bool tree, bool check_range, pwmhip_ws* outs, void* reuse,
-                        uint64_t reuse_bytes);
+                        uint64_t reuse_bytes, uint32_t flags);
 
 // Allocates ws->d_words for `levels` levels of `bits[l]` bits each and fills
 // the offsets.

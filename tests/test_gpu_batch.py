@@ -106,3 +106,40 @@ def test_batch_fallback_shapes_and_errors():
     assert e.value.status == pwm_amd.ERR_SYMBOL_RANGE
     with pytest.raises(pwm_amd.PwmError):
         pwm_amd.build_batch([np.zeros(3, np.uint8), np.zeros(3, np.uint16)], 3, "ppc")
+
+
+@pytest.mark.gpu
+def test_batch_graph_replay_and_autotune():
+    """Captured-graph replays and autotuned launch variants give the same
+    structures as the eager launch: the same geometry is replayed several
+    times (new text contents each time, eager builds in between), then a
+    batch of another geometry is captured anew."""
+    rng = np.random.default_rng(404)
+    sizes = np.clip(rng.lognormal(np.log(3000), 1.4, 600), 1, 300000).astype(np.int64)
+    sizes[::53] = 0
+    ctx = pwm_amd.Context(0)
+    try:
+        prev = None
+        for rep in range(4):
+            texts = [rng.integers(0, 5, n).astype(np.uint8) for n in sizes]
+            tree = rep == 3  # reps 1 and 2 replay the graph of rep 0
+            got = pwm_amd.build_batch(texts, 3, "ppc", tree, ctx=ctx, graph=True, autotune=True, reuse=prev)
+            eager = pwm_amd.build_batch(texts, 3, "ppc", tree, ctx=ctx)
+            for i in list(range(0, len(texts), 29)) + [int(np.argmax(sizes))]:
+                _same(got[i], eager[i], (rep, i))
+                if 0 < texts[i].size < 20000:
+                    o = O.naive(texts[i], 3, tree)
+                    for l, (x, y) in enumerate(zip(got[i].bvs(), o.levels)):
+                        assert np.array_equal(x, y), (rep, i, l)
+            del eager
+            prev = got
+        sizes2 = sizes[: len(sizes) // 2] + 7
+        texts = [rng.integers(0, 5, n).astype(np.uint8) for n in sizes2]
+        got = pwm_amd.build_batch(texts, 3, "ppc", False, ctx=ctx, graph=True, reuse=prev)
+        for i in range(0, len(texts), 17):
+            ref = pwm_amd.build(texts[i], 3, "ppc", False, ctx=ctx)
+            _same(got[i], ref, ("second geometry", i))
+            ref.free()
+        del got
+    finally:
+        ctx.close()
