@@ -832,6 +832,196 @@ __global__ __launch_bounds__(64 * 16) void k_local_small(K1Params p) {
   }
 }
 
+// ---- K1 over super-chunks ----------------------------------------------------
+// One text, all G (2..4) levels from the text.  The block's 16 waves take 16
+// consecutive chunks (a super-chunk of 64 Ki symbols) and the sub-levels are
+// assembled for the whole super-chunk in LDS: a symbol of bucket q in wave w
+// lands at (start of q in the super-chunk) + (q's count in waves < w) + (its
+// rank among the wave's q symbols).  The super-chunk's image is then written
+// out coalesced.  Runs of the stitch are 16 times longer than with one
+// chunk per run: fewer table entries and fewer cache lines shared by two
+// runs (each such line is fetched once per run).
+constexpr int kSuper = 16;                                  // chunks per super-chunk
+constexpr uint64_t kSuperSyms = uint64_t(kSuper) * kChunkSyms;
+constexpr uint32_t kSuperWords = kSuper * kChunkWords;      // words per sub-level
+
+template <int G>
+__global__ __launch_bounds__(64 * kSuper) void k_local_super(K1Params p) {
+  static_assert(G >= 2 && G <= 4, "two to four levels");
+  constexpr int WAVES = kSuper;
+  constexpr int NSUB = G - 1;      // sub-levels 1..G-1
+  constexpr int JC = G - 1;        // last sub-level
+  constexpr int NQ = 1 << JC;      // its buckets
+  constexpr int NF = NQ + 1;       // bucket counts + the last level's zeros
+  constexpr int NS = (NF + 1) / 2; // packed scans
+  __shared__ __align__(16) uint8_t lut[65536];
+  __shared__ uint64_t img[NSUB][kSuperWords];
+  __shared__ uint32_t wt[WAVES][NQ];
+  __shared__ unsigned long long red[G + 1];  // block sums: zeros per level, symbol OR
+  if (threadIdx.x <= G) red[threadIdx.x] = 0;
+  for (uint32_t i = threadIdx.x; i < NSUB * kSuperWords; i += 64 * WAVES) (&img[0][0])[i] = 0;
+
+  const int lane = threadIdx.x & 63;
+  const int wid = threadIdx.x >> 6;
+  const int uwid = __builtin_amdgcn_readfirstlane(wid);
+  load_lut<WAVES>(lut, p.lut);  // (ends with a barrier)
+
+  uint32_t zsum[G];
+#pragma unroll
+  for (int j = 0; j < G; ++j) zsum[j] = 0;
+  uint64_t my_or = 0;
+  const uint64_t nsuper = (p.nchunks + WAVES - 1) / WAVES;
+  const bool fast_mode = p.width == 1 && (reinterpret_cast<uintptr_t>(p.text) & 15) == 0;
+  auto full_chunk = [&](uint64_t c) { return fast_mode && c < p.nchunks && (c + 1) * kChunkSyms <= p.n; };
+  auto load_chunk = [&](uint64_t c, uint4* d) {
+    const auto src = gmem(reinterpret_cast<const uint4*>(static_cast<const uint8_t*>(p.text) + c * kChunkSyms +
+                                                         64ull * lane));
+    d[0] = src[0];
+    d[1] = src[1];
+    d[2] = src[2];
+    d[3] = src[3];
+  };
+  uint4 pf[4];
+  uint64_t sc = blockIdx.x;
+  if (sc < nsuper && full_chunk(sc * WAVES + wid)) load_chunk(sc * WAVES + wid, pf);
+
+  for (; sc < nsuper; sc += gridDim.x) {
+    const uint64_t c = sc * WAVES + wid;
+    const uint64_t cn = (sc + gridDim.x) * WAVES + wid;
+    const bool have = c < p.nchunks;
+    const uint32_t Ec = have ? uint32_t(min<uint64_t>(kChunkSyms, p.n - c * kChunkSyms)) : 0u;
+    const uint32_t nw = (Ec + 63) >> 6;
+    const int cnt = max(0, min(64, int(Ec) - 64 * lane));
+
+    uint64_t P0[G];
+#pragma unroll
+    for (int k = 0; k < G; ++k) P0[k] = 0;
+    if (full_chunk(c)) {
+      planes_from_bytes<G>(p, pf, P0, my_or);
+      if (full_chunk(cn)) load_chunk(cn, pf);  // prefetch
+    } else {
+      if (cnt > 0) {
+        const uint64_t e0 = c * kChunkSyms + 64ull * lane;
+        switch (p.width) {
+          case 1: load_planes_text<G, uint8_t>(p, p.text, e0, cnt, P0, my_or); break;
+          case 2: load_planes_text<G, uint16_t>(p, p.text, e0, cnt, P0, my_or); break;
+          case 4: load_planes_text<G, uint32_t>(p, p.text, e0, cnt, P0, my_or); break;
+          default: load_planes_text<G, uint64_t>(p, p.text, e0, cnt, P0, my_or); break;
+        }
+      }
+      if (full_chunk(cn)) load_chunk(cn, pf);
+    }
+    const uint64_t vm = cnt >= 64 ? ~0ull : (cnt <= 0 ? 0ull : ((1ull << cnt) - 1));
+    if (lane < int(nw)) gmem(p.out_level0)[c * kChunkWords + lane] = brev64(P0[0]);
+
+    // bucket masks of the last sub-level, M[q] = prod_i (bit i of q ? P_i : ~P_i)
+    uint64_t M[NQ];
+    M[0] = vm;
+#pragma unroll
+    for (int i = 0; i < JC; ++i) {
+#pragma unroll
+      for (int q = (1 << i) - 1; q >= 0; --q) {
+        M[q + (1 << i)] = M[q] & P0[i];
+        M[q] = M[q] & ~P0[i];
+      }
+    }
+    uint32_t f[2 * NS];
+#pragma unroll
+    for (int q = 0; q < NQ; ++q) f[q] = __popcll(M[q]);
+    f[NQ] = __popcll(~P0[G - 1] & vm);
+    if (NF < 2 * NS) f[2 * NS - 1] = 0;
+    uint32_t ex[2 * NS], tot[2 * NS];
+#pragma unroll
+    for (int s2 = 0; s2 < NS; ++s2) {
+      const uint32_t v = f[2 * s2] | (f[2 * s2 + 1] << 16);
+      const uint32_t inc = wave_incl_scan(v);
+      const uint32_t t = __builtin_amdgcn_readlane(inc, 63);
+      const uint32_t e = inc - v;
+      ex[2 * s2] = e & 0xffffu;
+      ex[2 * s2 + 1] = e >> 16;
+      tot[2 * s2] = t & 0xffffu;
+      tot[2 * s2 + 1] = t >> 16;
+    }
+#pragma unroll
+    for (int j = 0; j < JC; ++j) {
+      uint32_t z = 0;
+#pragma unroll
+      for (int q = 0; q < NQ; ++q)
+        if (!((q >> j) & 1)) z += tot[q];
+      zsum[j] += z;
+    }
+    zsum[G - 1] += tot[NQ];
+    if (lane < NQ) {
+      uint32_t v = 0;
+#pragma unroll
+      for (int q = 0; q < NQ; ++q) v = lane == q ? tot[q] : v;
+      wt[wid][lane] = v;
+    }
+    lds_sync();  // every wave's totals are there; the image was cleared
+
+    // the super-chunk's totals and this wave's prefix, per last-level bucket
+    uint32_t stot[NQ], spre[NQ];
+#pragma unroll
+    for (int q = 0; q < NQ; ++q) {
+      const uint32_t v = lane < WAVES ? wt[lane][q] : 0u;
+      const uint32_t inc = wave_incl_scan(v);
+      stot[q] = __builtin_amdgcn_readlane(inc, WAVES - 1);
+      spre[q] = __builtin_amdgcn_readlane(inc - v, uwid);
+    }
+    uint32_t cslot[2 * NQ - 2];
+#pragma unroll
+    for (int j = 1; j <= JC; ++j) {
+      const int nb = 1 << j;
+      uint32_t start = 0;
+#pragma unroll
+      for (int q = 0; q < nb; ++q) {
+        uint32_t T = 0, E = 0;
+#pragma unroll
+        for (int r = q; r < NQ; r += nb) {
+          T += stot[r];
+          E += spre[r] + ex[r];
+        }
+        uint64_t mq = vm;
+#pragma unroll
+        for (int i = 0; i < j; ++i) mq &= ((q >> i) & 1) ? P0[i] : ~P0[i];
+        or_bits(img[j - 1], start + E, compress64h(P0[j], mq, lut), __popcll(mq));
+        cslot[nb - 2 + q] = T;
+        start += T;
+      }
+    }
+    lds_sync();  // the image is complete
+
+    const uint64_t s0 = sc * kSuperSyms;
+    const uint32_t nws = uint32_t((min<uint64_t>(kSuperSyms, p.n - s0) + 63) >> 6);
+#pragma unroll
+    for (int j = 0; j < NSUB; ++j) {
+      if (threadIdx.x < nws)
+        gmem(p.temp_levels)[j * p.temp_stride + sc * kSuperWords + threadIdx.x] = brev64(img[j][threadIdx.x]);
+      img[j][threadIdx.x] = 0;  // cleared before the next super-chunk's first barrier
+    }
+    if (uwid == 0) {
+      uint32_t v = 0;
+#pragma unroll
+      for (int i = 0; i < 2 * NQ - 2; ++i) v = lane == i ? cslot[i] : v;
+      if (lane < 2 * NQ - 2) gmem(p.counts)[sc * p.nslot + lane] = v;
+    }
+  }
+  if (lane == 0) {
+#pragma unroll
+    for (int j = 0; j < G; ++j)
+      if (zsum[j]) atomicAdd(&red[j], (unsigned long long)zsum[j]);
+  }
+#pragma unroll
+  for (int d = 32; d >= 1; d >>= 1) my_or |= __shfl_xor(my_or, d, 64);
+  if (lane == 0 && my_or) atomicOr(&red[G], (unsigned long long)my_or);
+  __syncthreads();
+  if (threadIdx.x == 0) {
+    for (int j = 0; j < G; ++j)
+      if (red[j]) atomicAdd(p.zeros_out + j, red[j]);
+    if (red[G]) atomicOr(p.sym_or, red[G]);
+  }
+}
+
 // ---- symbol reorder between passes ------------------------------------------
 // After a pass over the top G levels of its symbols, the next pass reads the
 // symbols in the order of the level below: stable by bucket q = (bit of the
@@ -1431,7 +1621,7 @@ __global__ __launch_bounds__(1024) void k_apply_runs(const uint32_t* counts, con
                                                      const uint64_t* tnz, const uint64_t* gstart,
                                                      const uint64_t* nebase, uint64_t* tab_dst, uint64_t* tab_src,
                                                      uint64_t nchunks, uint32_t nslot, uint64_t ntiles,
-                                                     uint64_t* dstpos, uint32_t sc_slot0) {
+                                                     uint64_t* dstpos, uint32_t sc_slot0, uint64_t csyms) {
   __shared__ uint64_t wsum[2][16];
   const uint64_t tile = blockIdx.x;
   const uint32_t slot = blockIdx.y;
@@ -1471,7 +1661,7 @@ __global__ __launch_bounds__(1024) void k_apply_runs(const uint32_t* counts, con
   for (uint32_t b = s0; b < slot; ++b) loc += counts[c * nslot + b];
   const uint64_t e = uint64_t(s0) * nchunks + nebase[slot] + nzpref;
   tab_dst[e] = gstart[slot] + pref;
-  tab_src[e] = c * kChunkSyms + loc;
+  tab_src[e] = c * csyms + loc;
 }
 
 // ---- batch offsets --------------------------------------------------------
@@ -1637,6 +1827,23 @@ bool small_k1_disabled() {
     return e && e[0] == '1';
   }();
   return off;
+}
+
+// PWMHIP_CHUNK_RUNS=1: one run per chunk (no super-chunk assembly).
+bool super_disabled() {
+  static const bool off = [] {
+    const char* e = std::getenv("PWMHIP_CHUNK_RUNS");
+    return e && e[0] == '1';
+  }();
+  return off;
+}
+
+template <int G>
+void launch_super(pwmhip_ctx* ctx, const K1Params& prm) {
+  const uint64_t nsuper = (prm.nchunks + kSuper - 1) / kSuper;
+  const uint64_t grid = std::min<uint64_t>(nsuper, uint64_t(ctx->num_cus));
+  hipLaunchKernelGGL((k_local_super<G>), dim3(uint32_t(grid)), dim3(64 * kSuper), 0, ctx->stream, prm);
+  PWM_HIP_TRY(hipGetLastError());
 }
 
 template <int NP>
@@ -1862,7 +2069,7 @@ void build_passes_device(pwmhip_ctx* ctx, const void* d_text, uint64_t n, uint32
       hipLaunchKernelGGL(k_bucket_starts, dim3(jb), dim3(256), 0, s, U64(5), U64(7), 0, U64(8));
       hipLaunchKernelGGL(k_apply_runs, dim3(uint32_t(ntiles), nslot), dim3(1024), 0, s, counts, U64(3), U64(2),
                          U64(6), U64(7), U64(9), U64(10), nchunks, nslot, ntiles, U64(11),
-                         last ? nslot : (1u << g) - 2u);
+                         last ? nslot : (1u << g) - 2u, kChunkSyms);
       if (i == 0) PWM_HIP_TRY(hipEventRecord(ctx->ev[2], s));
       const uint64_t W = words_for(n);
       const uint32_t nwin = gather_windows(W, ctx->num_cus);
@@ -1938,6 +2145,11 @@ void build_plain_device(pwmhip_ctx* ctx, const void* d_text, uint64_t n, uint32_
   const uint64_t stride = level_stride(n);
   const bool multi = L > uint32_t(kMaxGroup);
   const bool tree_direct = tree && !multi;
+  // 2..4 levels from a text: runs per super-chunk; otherwise per chunk
+  const bool sup = L >= 2 && L <= 4 && !small_k1_disabled() && !super_disabled();
+  const uint64_t units = sup ? (nchunks + kSuper - 1) / kSuper : nchunks;
+  const uint64_t unit_syms = sup ? kSuperSyms : kChunkSyms;
+  const uint64_t tstride = sup ? units * kSuperWords + 64 : stride;
 
   // group plan
   std::vector<uint32_t> groups;
@@ -1948,12 +2160,12 @@ void build_plain_device(pwmhip_ctx* ctx, const void* d_text, uint64_t n, uint32_
   const uint32_t npay0 = L - gmax;
   const uint32_t nslot_max = (2u << gmax) - 2u;
   // Every scratch buffer is taken twice: once to size the arena, once for real.
-  const size_t sz_temp = size_t(std::max<uint32_t>(1, gmax - 1)) * stride;
+  const size_t sz_temp = size_t(std::max<uint32_t>(1, gmax - 1)) * tstride;
   const size_t sz_pay = std::max<size_t>(1, size_t(npay0) * stride);
   const size_t sz_pong = std::max<size_t>(1, size_t(npay0 > kMaxGroup ? npay0 - kMaxGroup : 0) * stride);
-  const size_t sz_cnt = std::max<size_t>(1, nchunks * nslot_max);
-  const size_t sz_tile = scan_scratch_words(nchunks, nslot_max);
-  const size_t sz_tab = std::max<size_t>(1, nchunks * nslot_max);
+  const size_t sz_cnt = std::max<size_t>(1, units * nslot_max);
+  const size_t sz_tile = scan_scratch_words(units, nslot_max);
+  const size_t sz_tab = std::max<size_t>(1, units * nslot_max);
   auto carve = [&](arena& A, uint64_t*& temp_levels, uint64_t*& temp_payload, uint64_t*& S_ping,
                    uint64_t*& S_pong, uint32_t*& counts, uint64_t*& pref, uint32_t*& ztot, uint64_t*& totals,
                    uint64_t*& gstart, uint64_t*& zeros_dev, unsigned long long*& sym_or, uint64_t*& tiles,
@@ -2020,17 +2232,26 @@ void build_plain_device(pwmhip_ctx* ctx, const void* d_text, uint64_t n, uint32_
     k1.out_level0 = ws->d_words + ws->level_offset[l0];
     k1.temp_levels = temp_levels;
     k1.temp_payload = temp_payload;
-    k1.temp_stride = stride;
+    k1.temp_stride = tstride;
     k1.counts = counts;
     k1.nslot = nslot;
     k1.zeros_out = reinterpret_cast<unsigned long long*>(zeros_dev + l0);
     k1.sym_or = sym_or;
     k1.nchunks = nchunks;
     k1.lut = compress_table(ctx);
-    dispatch_k1(ctx, k1);
+    if (sup) {
+      switch (L) {
+        case 2: launch_super<2>(ctx, k1); break;
+        case 3: launch_super<3>(ctx, k1); break;
+        default: launch_super<4>(ctx, k1); break;
+      }
+    } else {
+      dispatch_k1(ctx, k1);
+    }
     if (gi == 0) PWM_HIP_TRY(hipEventRecord(ctx->ev[1], s));
 
     if (nslot) {
+      const uint64_t nchunks = units;  // run units from here on
       const uint64_t ntiles = (nchunks + kTile - 1) / kTile;
       uint64_t* tsum = tiles;
       uint64_t* tnz = pref;  // tile sums of the indicators (pref is free scratch here)
@@ -2043,7 +2264,7 @@ void build_plain_device(pwmhip_ctx* ctx, const void* d_text, uint64_t n, uint32_
       hipLaunchKernelGGL(k_bucket_starts, dim3(jmax), dim3(256), 0, s, netot, nebase, tree_direct ? 1 : 0,
                          level_ne);
       hipLaunchKernelGGL(k_apply_runs, dim3(uint32_t(ntiles), nslot), dim3(1024), 0, s, counts, tsum, tnz, gstart,
-                         nebase, tab_dst, tab_src, nchunks, nslot, ntiles, (uint64_t*)nullptr, nslot);
+                         nebase, tab_dst, tab_src, nchunks, nslot, ntiles, (uint64_t*)nullptr, nslot, unit_syms);
       if (gi == 0) PWM_HIP_TRY(hipEventRecord(ctx->ev[2], s));
       const uint64_t W = words_for(n);
       const uint32_t nwin = gather_windows(W, ctx->num_cus);
@@ -2058,7 +2279,7 @@ void build_plain_device(pwmhip_ctx* ctx, const void* d_text, uint64_t n, uint32_
         gp.j = j;
         gp.n = n;
         if (j < g) {
-          gp.src = temp_levels + (j - 1) * stride;
+          gp.src = temp_levels + (j - 1) * tstride;
           gp.dst = ws->d_words + ws->level_offset[l0 + j];
           gp.nplanes = 1;
         } else {
