@@ -743,7 +743,7 @@ __global__ __launch_bounds__(64 * 16) void k_local_small(K1Params p) {
     uint32_t f[2 * NS];
 #pragma unroll
     for (int q = 0; q < NQ; ++q) f[q] = __popcll(M[q]);
-    if (!SC) f[NQ] = __popcll(~P0[G - 1] & vm);
+    if constexpr (!SC) f[NQ] = __popcll(~P0[G - 1] & vm);
     if (NF < 2 * NS) f[2 * NS - 1] = 0;
     uint32_t ex[2 * NS], tot[2 * NS];
 #pragma unroll
@@ -768,7 +768,7 @@ __global__ __launch_bounds__(64 * 16) void k_local_small(K1Params p) {
           if (!((q >> j) & 1)) z += tot[q];
         zsum[j] += z;
       }
-      if (!SC) zsum[G - 1] += tot[NQ];
+      if constexpr (!SC) zsum[G - 1] += tot[NQ];
     }
 
     // sub-level j (1..G-1): buckets q < 2^j; counts/prefixes are sums over
@@ -1537,6 +1537,140 @@ __device__ __forceinline__ void gather_range(const uint64_t* tab_dst, const uint
   }
 }
 
+// Two-word lanes: destination steps of 128 words, lane i owns words 2i and
+// 2i+1 (16-byte loads and stores).  A step inside one run is a shift copy
+// with one source offset for the whole wave; four steps issue their loads
+// together.  Steps that a run boundary crosses go through gather_window
+// (64-word windows, one word per lane).
+typedef uint64_t u64x2_u __attribute__((ext_vector_type(2), aligned(8)));
+
+__device__ __forceinline__ void gather_range_wide(const uint64_t* tab_dst, const uint64_t* tab_src, uint64_t NE,
+                                                  const uint64_t* src, uint64_t* dst, uint64_t n, uint64_t W0,
+                                                  uint32_t nstep, uint64_t first, unsigned long long* stage) {
+  constexpr uint32_t kSteps = 4;
+  const uint32_t lane = threadIdx.x & 63;
+  const uint64_t X = W0 << 6;
+  if (X >= n || NE == 0) return;
+  RunCache rc{tab_dst, tab_src, NE, n, first, 0, 0};  // first: last entry with dst <= X
+  rc.load(lane);
+  for (uint32_t g0 = 0; g0 < nstep; g0 += kSteps) {
+    const uint64_t gw0 = W0 + 128ull * g0;
+    if ((gw0 << 6) >= n) break;
+    const uint64_t gend = min<uint64_t>((gw0 + 128ull * kSteps) << 6, n);
+    uint32_t ef = __popcll(__ballot(rc.bd <= (gw0 << 6))) - 1;
+    uint32_t el = __popcll(__ballot(rc.bd < gend)) - 1;
+    if (el == 63 && ef > 0) {
+      rc.base += ef;
+      rc.load(lane);
+      el = __popcll(__ballot(rc.bd < gend)) - 1;
+    }
+    // steps inside one cached run (whose end is cached too)
+    uint32_t single = 0;
+    int64_t dl[kSteps];
+#pragma unroll
+    for (int k = 0; k < int(kSteps); ++k) {
+      const uint64_t sw0 = (gw0 + 128ull * k) << 6, sw1 = min<uint64_t>(sw0 + 8192, n);
+      dl[k] = 0;
+      if (sw0 < n && el < 63) {
+        const uint32_t a = __popcll(__ballot(rc.bd <= sw0)) - 1;
+        const uint32_t b = __popcll(__ballot(rc.bd < sw1)) - 1;
+        if (a == b) {
+          single |= 1u << k;
+          dl[k] = int64_t(readlane64(rc.bs, a)) - int64_t(readlane64(rc.bd, a));
+        }
+      }
+    }
+    u64x2_u lo2[kSteps];
+    uint64_t hi1[kSteps];
+#pragma unroll
+    for (int k = 0; k < int(kSteps); ++k) {
+      const uint64_t ws = ((gw0 + 128ull * k) << 6) + 128ull * lane;
+      if (((single >> k) & 1) && ws < n) {
+        const uint64_t q = uint64_t(dl[k] + int64_t(ws)) >> 6;
+        lo2[k] = *reinterpret_cast<const u64x2_u*>(gmem(src) + q);
+        hi1[k] = gmem(src)[q + 2];
+      }
+    }
+#pragma unroll
+    for (int k = 0; k < int(kSteps); ++k) {
+      const uint64_t ws = ((gw0 + 128ull * k) << 6) + 128ull * lane;
+      if (((single >> k) & 1) && ws < n) {
+        const uint32_t sh = uint32_t(uint64_t(dl[k] + int64_t(ws)) & 63);
+        uint64_t v0 = lo2[k].x, v1 = lo2[k].y;
+        if (sh) {
+          v0 = (v0 << sh) | (lo2[k].y >> (64 - sh));
+          v1 = (v1 << sh) | (hi1[k] >> (64 - sh));
+        }
+        uint64_t* d = gmem(dst) + (ws >> 6);
+        if (ws + 128 <= n) {
+          u64x2_u o;
+          o.x = v0;
+          o.y = v1;
+          *reinterpret_cast<u64x2_u*>(d) = o;
+        } else {  // the level's end: bits from n on are zero
+          if (ws + 64 < n) {
+            d[0] = v0;
+            d[1] = v1 & range_mask(0, uint32_t(n - ws - 64));
+          } else {
+            d[0] = v0 & range_mask(0, uint32_t(n - ws));
+          }
+        }
+      }
+    }
+#pragma unroll 1
+    for (uint32_t k = 0; k < kSteps; ++k) {
+      if ((single >> k) & 1) continue;
+      const uint64_t w0 = gw0 + 128ull * k;
+      if ((w0 << 6) >= n) break;
+      gather_window(src, dst, rc, w0, stage);
+      if (((w0 + 64) << 6) < n) gather_window(src, dst, rc, w0 + 64, stage);
+    }
+  }
+}
+
+// Every sub-level of a pass in one grid (blockIdx.y = sub-level - 1).
+struct WideParams {
+  const uint64_t* tab_dst;   // level j's entries from ((1 << j) - 2) * units
+  const uint64_t* tab_src;
+  const uint64_t* level_ne;  // entries per sub-level (device, index j)
+  uint32_t* first;           // [(j - 1) * wpl + wave]: the wave's first entry
+  const uint64_t* src;       // sub-level j at src + (j - 1) * src_stride
+  uint64_t src_stride;
+  uint64_t* dst[kMaxGroup];  // destination level of sub-level j (index j)
+  uint64_t units, n, wpl;    // run units, symbols, waves per level
+  uint32_t nstep;
+};
+
+// First table entry of every wave of the wide stitch: the last entry whose
+// destination start is at or before the wave's first bit.
+__global__ __launch_bounds__(256) void k_wave_starts(WideParams p) {
+  const uint64_t k = uint64_t(blockIdx.x) * 256 + threadIdx.x;
+  const uint32_t j = blockIdx.y + 1;
+  if (k >= p.wpl) return;
+  const uint64_t X = k * 128ull * p.nstep * 64;
+  const uint64_t* td = p.tab_dst + uint64_t((1u << j) - 2) * p.units;
+  const uint64_t NE = p.level_ne[j];
+  uint64_t lo = 0, hi = NE;  // td[lo] <= X < td[hi] (td[0] == 0)
+  if (X < p.n && NE > 0) {
+    while (hi - lo > 1) {
+      const uint64_t mid = (lo + hi) >> 1;
+      if (td[mid] <= X) lo = mid;
+      else hi = mid;
+    }
+  }
+  p.first[(j - 1) * p.wpl + k] = uint32_t(lo);
+}
+
+__global__ __launch_bounds__(256) void k_gather_wide(WideParams p) {
+  __shared__ unsigned long long stage[4][64];
+  const uint32_t j = blockIdx.y + 1;
+  const uint64_t wave = uint64_t(blockIdx.x) * 4 + (threadIdx.x >> 6);
+  if (wave >= p.wpl) return;  // whole waves; no barrier below
+  const uint64_t toff = uint64_t((1u << j) - 2) * p.units;
+  gather_range_wide(p.tab_dst + toff, p.tab_src + toff, p.level_ne[j], p.src + (j - 1) * p.src_stride, p.dst[j],
+                    p.n, wave * 128 * p.nstep, p.nstep, p.first[(j - 1) * p.wpl + wave], stage[threadIdx.x >> 6]);
+}
+
 __global__ __launch_bounds__(256) void k_gather(GatherParams p) {
   __shared__ unsigned long long stage[4][64];
   const uint64_t W0 = (uint64_t(blockIdx.x) * 4 + (threadIdx.x >> 6)) * 64 * p.nwin;
@@ -1836,6 +1970,25 @@ bool super_disabled() {
     return e && e[0] == '1';
   }();
   return off;
+}
+
+// PWMHIP_NARROW_GATHER=1: one destination word per lane in the stitch.
+bool wide_gather_disabled() {
+  static const bool off = [] {
+    const char* e = std::getenv("PWMHIP_NARROW_GATHER");
+    return e && e[0] == '1';
+  }();
+  return off;
+}
+
+// 128-word steps per wave of the wide stitch (multiple of 4; PWMHIP_WIDE_STEPS).
+uint32_t wide_gather_steps() {
+  static const uint32_t v = [] {
+    const char* e = std::getenv("PWMHIP_WIDE_STEPS");
+    const long x = e ? std::atol(e) : 8;
+    return uint32_t(x >= 4 && x <= 256 ? (x + 3) / 4 * 4 : 8);
+  }();
+  return v;
 }
 
 template <int G>
@@ -2150,6 +2303,10 @@ void build_plain_device(pwmhip_ctx* ctx, const void* d_text, uint64_t n, uint32_
   const uint64_t units = sup ? (nchunks + kSuper - 1) / kSuper : nchunks;
   const uint64_t unit_syms = sup ? kSuperSyms : kChunkSyms;
   const uint64_t tstride = sup ? units * kSuperWords + 64 : stride;
+  // wide stitch: 128-word steps, wide_nstep per wave, one grid for all sub-levels
+  const bool wide = sup && !wide_gather_disabled();
+  const uint32_t wide_nstep = wide_gather_steps();
+  const uint64_t wide_wpl = (words_for(n) + 128ull * wide_nstep - 1) / (128ull * wide_nstep);
 
   // group plan
   std::vector<uint32_t> groups;
@@ -2166,6 +2323,7 @@ void build_plain_device(pwmhip_ctx* ctx, const void* d_text, uint64_t n, uint32_
   const size_t sz_cnt = std::max<size_t>(1, units * nslot_max);
   const size_t sz_tile = scan_scratch_words(units, nslot_max);
   const size_t sz_tab = std::max<size_t>(1, units * nslot_max);
+  uint32_t* wave_first = nullptr;
   auto carve = [&](arena& A, uint64_t*& temp_levels, uint64_t*& temp_payload, uint64_t*& S_ping,
                    uint64_t*& S_pong, uint32_t*& counts, uint64_t*& pref, uint32_t*& ztot, uint64_t*& totals,
                    uint64_t*& gstart, uint64_t*& zeros_dev, unsigned long long*& sym_or, uint64_t*& tiles,
@@ -2189,6 +2347,7 @@ void build_plain_device(pwmhip_ctx* ctx, const void* d_text, uint64_t n, uint32_
     level_ne = A.take<uint64_t>(kMaxGroup + 1);
     tab_dst = A.take<uint64_t>(sz_tab);
     tab_src = A.take<uint64_t>(sz_tab);
+    wave_first = A.take<uint32_t>(std::max<size_t>(1, wide ? size_t(gmax) * wide_wpl : 1));
   };
   uint64_t *temp_levels, *temp_payload, *S_ping, *S_pong, *pref, *totals, *gstart, *zeros_dev, *tiles;
   uint64_t *nepref, *netot, *nebase, *level_ne, *tab_dst, *tab_src;
@@ -2265,6 +2424,17 @@ void build_plain_device(pwmhip_ctx* ctx, const void* d_text, uint64_t n, uint32_
                          level_ne);
       hipLaunchKernelGGL(k_apply_runs, dim3(uint32_t(ntiles), nslot), dim3(1024), 0, s, counts, tsum, tnz, gstart,
                          nebase, tab_dst, tab_src, nchunks, nslot, ntiles, (uint64_t*)nullptr, nslot, unit_syms);
+      if (wide) {
+        WideParams wp{};
+        wp.tab_dst = tab_dst;
+        wp.level_ne = level_ne;
+        wp.first = wave_first;
+        wp.units = nchunks;
+        wp.n = n;
+        wp.wpl = wide_wpl;
+        wp.nstep = wide_nstep;
+        hipLaunchKernelGGL(k_wave_starts, dim3(uint32_t((wide_wpl + 255) / 256), jmax), dim3(256), 0, s, wp);
+      }
       if (gi == 0) PWM_HIP_TRY(hipEventRecord(ctx->ev[2], s));
       const uint64_t W = words_for(n);
       const uint32_t nwin = gather_windows(W, ctx->num_cus);
@@ -2289,7 +2459,22 @@ void build_plain_device(pwmhip_ctx* ctx, const void* d_text, uint64_t n, uint32_
           gp.dst_stride = stride;
           gp.nplanes = npay;
         }
-        hipLaunchKernelGGL(k_gather, dim3(gblocks), dim3(256), 0, s, gp);
+        if (!wide) hipLaunchKernelGGL(k_gather, dim3(gblocks), dim3(256), 0, s, gp);
+      }
+      if (wide) {
+        WideParams wp{};
+        wp.tab_dst = tab_dst;
+        wp.tab_src = tab_src;
+        wp.level_ne = level_ne;
+        wp.first = wave_first;
+        wp.src = temp_levels;
+        wp.src_stride = tstride;
+        for (uint32_t j = 1; j <= jmax; ++j) wp.dst[j] = ws->d_words + ws->level_offset[l0 + j];
+        wp.units = nchunks;
+        wp.n = n;
+        wp.wpl = wide_wpl;
+        wp.nstep = wide_nstep;
+        hipLaunchKernelGGL(k_gather_wide, dim3(uint32_t((wide_wpl + 3) / 4), jmax), dim3(256), 0, s, wp);
       }
     } else if (gi == 0) {
       PWM_HIP_TRY(hipEventRecord(ctx->ev[2], s));
