@@ -1798,6 +1798,102 @@ __global__ __launch_bounds__(1024) void k_apply_runs(const uint32_t* counts, con
   tab_src[e] = c * csyms + loc;
 }
 
+// The same run tables in one kernel after k_tile_sums2, for passes of few
+// buckets and tiles (every block scans the unscanned tile sums it needs):
+// the bucket starts of the block's level, its tile's prefix, the entries,
+// the entry counts per level and the first entry of every wave of the wide
+// stitch (a run [d, d + x) holds the wave starts k * wave_bits inside it).
+struct FusedRuns {
+  const uint32_t* counts;
+  const uint64_t* tsum;  // [slot * ntiles + tile], unscanned
+  const uint64_t* tnz;
+  uint64_t* tab_dst;
+  uint64_t* tab_src;
+  uint64_t* level_ne;
+  uint32_t* wave_first;  // [(j - 1) * wpl + k]
+  uint64_t units, ntiles, csyms, wave_bits, wpl;
+  uint32_t nslot;
+  int tree;
+};
+
+__global__ __launch_bounds__(1024) void k_runs_fused(FusedRuns p) {
+  __shared__ uint64_t wsum[2][16];
+  __shared__ uint64_t base[2];  // bucket start, entry start (this slot, this tile)
+  const uint64_t tile = blockIdx.x;
+  const uint32_t slot = blockIdx.y;
+  const uint32_t j = slot_level(slot), s0 = (1u << j) - 2, nb = 1u << j, b = slot - s0;
+  const int t = threadIdx.x, lane = t & 63, wid = t >> 6;
+  if (wid == 0) {
+    // totals of the level's buckets before this one (output order) and the
+    // sums of this slot's earlier tiles
+    const uint32_t pb = p.tree ? rev_bits(b, j) : b;
+    uint64_t gv = 0, gz = 0, lz = 0, pv = 0, pz = 0;
+    for (uint32_t bb = 0; bb < nb; ++bb) {
+      const uint32_t pos = p.tree ? rev_bits(bb, j) : bb;
+      const uint64_t* sv = p.tsum + uint64_t(s0 + bb) * p.ntiles;
+      const uint64_t* sz = p.tnz + uint64_t(s0 + bb) * p.ntiles;
+      for (uint64_t i = lane; i < p.ntiles; i += 64) {
+        const uint64_t v = sv[i], z = sz[i];
+        lz += z;
+        if (pos < pb) {
+          gv += v;
+          gz += z;
+        }
+        if (bb == b && i < tile) {
+          pv += v;
+          pz += z;
+        }
+      }
+    }
+#pragma unroll
+    for (int d = 32; d >= 1; d >>= 1) {
+      gv += __shfl_xor(gv, d, 64);
+      gz += __shfl_xor(gz, d, 64);
+      lz += __shfl_xor(lz, d, 64);
+      pv += __shfl_xor(pv, d, 64);
+      pz += __shfl_xor(pz, d, 64);
+    }
+    if (lane == 0) {
+      base[0] = gv + pv;
+      base[1] = gz + pz;
+      if (tile == 0 && b == 0) p.level_ne[j] = lz;
+    }
+  }
+  const uint64_t c = tile * kTile + t;
+  const uint32_t x = c < p.units ? p.counts[c * p.nslot + slot] : 0;
+  uint64_t iv = x, iz = x != 0;
+#pragma unroll
+  for (int d = 1; d < 64; d <<= 1) {
+    const uint64_t a = __shfl_up(iv, d, 64), bz = __shfl_up(iz, d, 64);
+    if (lane >= d) {
+      iv += a;
+      iz += bz;
+    }
+  }
+  if (lane == 63) {
+    wsum[0][wid] = iv;
+    wsum[1][wid] = iz;
+  }
+  __syncthreads();
+  if (!x) return;
+  uint64_t ov = base[0], oz = base[1];
+#pragma unroll
+  for (int w = 0; w < 16; ++w)
+    if (w < wid) {
+      ov += wsum[0][w];
+      oz += wsum[1][w];
+    }
+  const uint64_t d = ov + iv - x, e = oz + iz - 1;  // destination start, entry in the level
+  uint64_t loc = 0;
+  for (uint32_t bb = s0; bb < slot; ++bb) loc += p.counts[c * p.nslot + bb];
+  p.tab_dst[uint64_t(s0) * p.units + e] = d;
+  p.tab_src[uint64_t(s0) * p.units + e] = c * p.csyms + loc;
+  if (p.wave_first) {
+    for (uint64_t k = (d + p.wave_bits - 1) / p.wave_bits; k * p.wave_bits < d + x && k < p.wpl; ++k)
+      p.wave_first[uint64_t(j - 1) * p.wpl + k] = uint32_t(e);
+  }
+}
+
 // ---- batch offsets --------------------------------------------------------
 // In-tile inclusive scans of one slot's counts and non-empty indicators over
 // the tile's 1024 chunks (one thread per chunk), plus the tile's offsets:
@@ -2416,24 +2512,43 @@ void build_plain_device(pwmhip_ctx* ctx, const void* d_text, uint64_t n, uint32_
       uint64_t* tnz = pref;  // tile sums of the indicators (pref is free scratch here)
       hipLaunchKernelGGL(k_tile_sums2, dim3(uint32_t(ntiles), nslot), dim3(256), 0, s, counts, tsum, tnz, nchunks,
                          nslot, ntiles);
-      hipLaunchKernelGGL(k_tile_scan, dim3(nslot), dim3(1024), 0, s, tsum, totals, ntiles);
-      hipLaunchKernelGGL(k_tile_scan, dim3(nslot), dim3(1024), 0, s, tnz, netot, ntiles);
-      hipLaunchKernelGGL(k_bucket_starts, dim3(jmax), dim3(256), 0, s, totals, gstart, tree_direct ? 1 : 0,
-                         (uint64_t*)nullptr);
-      hipLaunchKernelGGL(k_bucket_starts, dim3(jmax), dim3(256), 0, s, netot, nebase, tree_direct ? 1 : 0,
-                         level_ne);
-      hipLaunchKernelGGL(k_apply_runs, dim3(uint32_t(ntiles), nslot), dim3(1024), 0, s, counts, tsum, tnz, gstart,
-                         nebase, tab_dst, tab_src, nchunks, nslot, ntiles, (uint64_t*)nullptr, nslot, unit_syms);
-      if (wide) {
-        WideParams wp{};
-        wp.tab_dst = tab_dst;
-        wp.level_ne = level_ne;
-        wp.first = wave_first;
-        wp.units = nchunks;
-        wp.n = n;
-        wp.wpl = wide_wpl;
-        wp.nstep = wide_nstep;
-        hipLaunchKernelGGL(k_wave_starts, dim3(uint32_t((wide_wpl + 255) / 256), jmax), dim3(256), 0, s, wp);
+      if (wide && ntiles <= 1024) {
+        FusedRuns fr{};
+        fr.counts = counts;
+        fr.tsum = tsum;
+        fr.tnz = tnz;
+        fr.tab_dst = tab_dst;
+        fr.tab_src = tab_src;
+        fr.level_ne = level_ne;
+        fr.wave_first = wave_first;
+        fr.units = nchunks;
+        fr.ntiles = ntiles;
+        fr.csyms = unit_syms;
+        fr.wave_bits = 128ull * 64 * wide_nstep;
+        fr.wpl = wide_wpl;
+        fr.nslot = nslot;
+        fr.tree = tree_direct ? 1 : 0;
+        hipLaunchKernelGGL(k_runs_fused, dim3(uint32_t(ntiles), nslot), dim3(1024), 0, s, fr);
+      } else {
+        hipLaunchKernelGGL(k_tile_scan, dim3(nslot), dim3(1024), 0, s, tsum, totals, ntiles);
+        hipLaunchKernelGGL(k_tile_scan, dim3(nslot), dim3(1024), 0, s, tnz, netot, ntiles);
+        hipLaunchKernelGGL(k_bucket_starts, dim3(jmax), dim3(256), 0, s, totals, gstart, tree_direct ? 1 : 0,
+                           (uint64_t*)nullptr);
+        hipLaunchKernelGGL(k_bucket_starts, dim3(jmax), dim3(256), 0, s, netot, nebase, tree_direct ? 1 : 0,
+                           level_ne);
+        hipLaunchKernelGGL(k_apply_runs, dim3(uint32_t(ntiles), nslot), dim3(1024), 0, s, counts, tsum, tnz, gstart,
+                           nebase, tab_dst, tab_src, nchunks, nslot, ntiles, (uint64_t*)nullptr, nslot, unit_syms);
+        if (wide) {
+          WideParams wp{};
+          wp.tab_dst = tab_dst;
+          wp.level_ne = level_ne;
+          wp.first = wave_first;
+          wp.units = nchunks;
+          wp.n = n;
+          wp.wpl = wide_wpl;
+          wp.nstep = wide_nstep;
+          hipLaunchKernelGGL(k_wave_starts, dim3(uint32_t((wide_wpl + 255) / 256), jmax), dim3(256), 0, s, wp);
+        }
       }
       if (gi == 0) PWM_HIP_TRY(hipEventRecord(ctx->ev[2], s));
       const uint64_t W = words_for(n);
