@@ -155,6 +155,48 @@ __device__ __forceinline__ auto gmem(T* p) {
 #endif
 }
 
+// Streaming accesses.  Stores of data written once (levels, chunk-local
+// rows) carry the non-temporal hint; measured on the 4 GiB DNA build, the
+// hint helps stores (-1.5%) and hurts the text loads (+40%), so loads stay
+// ordinary.
+template <typename T>
+__device__ __forceinline__ T ld_stream(const T* p) {
+  return *gmem(p);
+}
+
+template <typename T>
+__device__ __forceinline__ void st_stream(T* p, T v) {
+#if __HIP_DEVICE_COMPILE__
+  __builtin_nontemporal_store(v, gmem(p));
+#else
+  *p = v;
+#endif
+}
+
+typedef uint32_t u32x4_t __attribute__((ext_vector_type(4)));
+typedef uint64_t u64x2_u __attribute__((ext_vector_type(2), aligned(8)));
+
+// Two consecutive words at an 8-byte aligned address.
+#if __HIP_DEVICE_COMPILE__
+typedef __attribute__((address_space(1))) const u64x2_u gc_u64x2;
+typedef __attribute__((address_space(1))) u64x2_u g_u64x2;
+#else
+typedef const u64x2_u gc_u64x2;
+typedef u64x2_u g_u64x2;
+#endif
+
+__device__ __forceinline__ u64x2_u ld_stream2(const uint64_t* p) {
+  return *(gc_u64x2*)p;
+}
+
+__device__ __forceinline__ void st_stream2(uint64_t* p, u64x2_u v) {
+#if __HIP_DEVICE_COMPILE__
+  __builtin_nontemporal_store(v, (g_u64x2*)p);
+#else
+  *(g_u64x2*)p = v;
+#endif
+}
+
 template <typename T>
 __device__ __forceinline__ uint64_t ld_sym(const void* text, uint64_t i) {
   return static_cast<const T*>(text)[i];
@@ -725,7 +767,7 @@ __global__ __launch_bounds__(64 * 16) void k_local_small(K1Params p) {
       if (cn < p.nchunks && fast_mode && full_chunk(nv)) load_chunk(nv, pf);
     }
     const uint64_t vm = cnt >= 64 ? ~0ull : (cnt <= 0 ? 0ull : ((1ull << cnt) - 1));
-    if (lane < int(nw)) gmem(cv.lvl0)[lane] = brev64(P0[0]);
+    if (lane < int(nw)) st_stream(cv.lvl0 + lane, brev64(P0[0]));
 #pragma unroll
     for (int j = 0; j < NSUB; ++j) buf[j][lane] = 0;
 
@@ -801,7 +843,7 @@ __global__ __launch_bounds__(64 * 16) void k_local_small(K1Params p) {
     if (lane < int(nw)) {
 #pragma unroll
       for (int j = 1; j < G; ++j)
-        gmem(p.temp_levels)[(j - 1) * p.temp_stride + c * P + lane] = brev64(buf[j - 1][lane]);
+        st_stream(p.temp_levels + (j - 1) * p.temp_stride + c * P + lane, brev64(buf[j - 1][lane]));
     }
     if (G > 1) {
       uint32_t v = 0;
@@ -874,12 +916,13 @@ __global__ __launch_bounds__(64 * kSuper) void k_local_super(K1Params p) {
   const bool fast_mode = p.width == 1 && (reinterpret_cast<uintptr_t>(p.text) & 15) == 0;
   auto full_chunk = [&](uint64_t c) { return fast_mode && c < p.nchunks && (c + 1) * kChunkSyms <= p.n; };
   auto load_chunk = [&](uint64_t c, uint4* d) {
-    const auto src = gmem(reinterpret_cast<const uint4*>(static_cast<const uint8_t*>(p.text) + c * kChunkSyms +
-                                                         64ull * lane));
-    d[0] = src[0];
-    d[1] = src[1];
-    d[2] = src[2];
-    d[3] = src[3];
+    const u32x4_t* src = reinterpret_cast<const u32x4_t*>(static_cast<const uint8_t*>(p.text) + c * kChunkSyms +
+                                                          64ull * lane);
+#pragma unroll
+    for (int i = 0; i < 4; ++i) {
+      const u32x4_t v = ld_stream(src + i);
+      d[i] = make_uint4(v.x, v.y, v.z, v.w);
+    }
   };
   uint4 pf[4];
   uint64_t sc = blockIdx.x;
@@ -912,7 +955,7 @@ __global__ __launch_bounds__(64 * kSuper) void k_local_super(K1Params p) {
       if (full_chunk(cn)) load_chunk(cn, pf);
     }
     const uint64_t vm = cnt >= 64 ? ~0ull : (cnt <= 0 ? 0ull : ((1ull << cnt) - 1));
-    if (lane < int(nw)) gmem(p.out_level0)[c * kChunkWords + lane] = brev64(P0[0]);
+    if (lane < int(nw)) st_stream(p.out_level0 + c * kChunkWords + lane, brev64(P0[0]));
 
     // bucket masks of the last sub-level, M[q] = prod_i (bit i of q ? P_i : ~P_i)
     uint64_t M[NQ];
@@ -996,7 +1039,7 @@ __global__ __launch_bounds__(64 * kSuper) void k_local_super(K1Params p) {
 #pragma unroll
     for (int j = 0; j < NSUB; ++j) {
       if (threadIdx.x < nws)
-        gmem(p.temp_levels)[j * p.temp_stride + sc * kSuperWords + threadIdx.x] = brev64(img[j][threadIdx.x]);
+        st_stream(p.temp_levels + j * p.temp_stride + sc * kSuperWords + threadIdx.x, brev64(img[j][threadIdx.x]));
       img[j][threadIdx.x] = 0;  // cleared before the next super-chunk's first barrier
     }
     if (uwid == 0) {
@@ -1399,7 +1442,7 @@ __device__ void gather_window_runs(const uint64_t* src, uint64_t* dst, RunCache&
     }
   }
   wave_sync();
-  if (((w0 + lane) << 6) < n) gmem(dst)[w0 + lane] = stage[lane];
+  if (((w0 + lane) << 6) < n) st_stream(dst + w0 + lane, uint64_t(stage[lane]));
   rc.base = e1;  // the window's last run may continue into the next window
   rc.load(lane);
 }
@@ -1438,7 +1481,7 @@ __device__ __forceinline__ void gather_window(const uint64_t* src, uint64_t* dst
     bool pending = live;  // every run of the window is in the cache
     v = gather_word(src, rc.bd, rc.bs, ws, we, pending);
   }
-  if (live) gmem(dst)[w0 + lane] = v;
+  if (live) st_stream(dst + w0 + lane, v);
 }
 
 // Writes the wave's destination words [W0, W0 + 64*nwin) of `nplanes`
@@ -1520,7 +1563,7 @@ __device__ __forceinline__ void gather_range(const uint64_t* tab_dst, const uint
                 ++e;
               }
             }
-            if (ws < n) gmem(dst)[ws >> 6] = v[k];
+            if (ws < n) st_stream(dst + (ws >> 6), v[k]);
           }
           done = true;
         }
@@ -1542,8 +1585,6 @@ __device__ __forceinline__ void gather_range(const uint64_t* tab_dst, const uint
 // with one source offset for the whole wave; four steps issue their loads
 // together.  Steps that a run boundary crosses go through gather_window
 // (64-word windows, one word per lane).
-typedef uint64_t u64x2_u __attribute__((ext_vector_type(2), aligned(8)));
-
 __device__ __forceinline__ void gather_range_wide(const uint64_t* tab_dst, const uint64_t* tab_src, uint64_t NE,
                                                   const uint64_t* src, uint64_t* dst, uint64_t n, uint64_t W0,
                                                   uint32_t nstep, uint64_t first, unsigned long long* stage) {
@@ -1587,7 +1628,7 @@ __device__ __forceinline__ void gather_range_wide(const uint64_t* tab_dst, const
       const uint64_t ws = ((gw0 + 128ull * k) << 6) + 128ull * lane;
       if (((single >> k) & 1) && ws < n) {
         const uint64_t q = uint64_t(dl[k] + int64_t(ws)) >> 6;
-        lo2[k] = *reinterpret_cast<const u64x2_u*>(gmem(src) + q);
+        lo2[k] = ld_stream2(src + q);
         hi1[k] = gmem(src)[q + 2];
       }
     }
@@ -1601,18 +1642,18 @@ __device__ __forceinline__ void gather_range_wide(const uint64_t* tab_dst, const
           v0 = (v0 << sh) | (lo2[k].y >> (64 - sh));
           v1 = (v1 << sh) | (hi1[k] >> (64 - sh));
         }
-        uint64_t* d = gmem(dst) + (ws >> 6);
+        uint64_t* d = dst + (ws >> 6);
         if (ws + 128 <= n) {
           u64x2_u o;
           o.x = v0;
           o.y = v1;
-          *reinterpret_cast<u64x2_u*>(d) = o;
+          st_stream2(d, o);
         } else {  // the level's end: bits from n on are zero
           if (ws + 64 < n) {
-            d[0] = v0;
-            d[1] = v1 & range_mask(0, uint32_t(n - ws - 64));
+            gmem(d)[0] = v0;
+            gmem(d)[1] = v1 & range_mask(0, uint32_t(n - ws - 64));
           } else {
-            d[0] = v0 & range_mask(0, uint32_t(n - ws));
+            gmem(d)[0] = v0 & range_mask(0, uint32_t(n - ws));
           }
         }
       }
