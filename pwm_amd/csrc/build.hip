@@ -296,21 +296,7 @@ __device__ __forceinline__ void load_planes_text(const K1Params& p, const void* 
                                                  uint64_t* acc, uint64_t& my_or) {
   // symbol bit of plane k: s_k = L-1-l0-k ; byte index s_k>>3, bit s_k&7
   const auto tx = gmem(static_cast<const T*>(text));
-#pragma unroll 1
-  for (int q = 0; q < 16; ++q) {
-    uint64_t v[4];
-    if (cnt >= 4 * q + 4) {
-      if constexpr (sizeof(T) == 1) {
-        const uint32_t x = *gmem(reinterpret_cast<const uint32_t*>(static_cast<const T*>(text) + e0 + 4 * q));
-        v[0] = x & 0xff; v[1] = (x >> 8) & 0xff; v[2] = (x >> 16) & 0xff; v[3] = x >> 24;
-      } else {
-#pragma unroll
-        for (int e = 0; e < 4; ++e) v[e] = tx[e0 + 4 * q + e];
-      }
-    } else {
-#pragma unroll
-      for (int e = 0; e < 4; ++e) v[e] = (4 * q + e < cnt) ? uint64_t(tx[e0 + 4 * q + e]) : 0;
-    }
+  auto planes4 = [&](const uint64_t* v, int q) {
     my_or |= v[0] | v[1] | v[2] | v[3];
     uint32_t cur_bi = 0xffffffffu, x4 = 0;
 #pragma unroll
@@ -326,6 +312,57 @@ __device__ __forceinline__ void load_planes_text(const K1Params& p, const void* 
         acc[k] |= uint64_t(gather4(x4, s & 7)) << (4 * q);
       }
     }
+  };
+  if constexpr (sizeof(T) >= 2) {
+    // a full lane at a 16-byte aligned address: its 64 symbols in batches of
+    // 64 bytes, each batch's four 16-byte loads issued together
+    const T* lp = static_cast<const T*>(text) + e0;
+    if (cnt == 64 && (reinterpret_cast<uintptr_t>(lp) & 15) == 0) {
+      constexpr int SB = 64 / int(sizeof(T));  // symbols per batch
+      const auto g4 = gmem(reinterpret_cast<const uint4*>(lp));
+#pragma unroll 1
+      for (int b = 0; b < 64 / SB; ++b) {
+        uint32_t u[16];
+#pragma unroll
+        for (int i = 0; i < 4; ++i) {
+          const uint4 w = g4[4 * b + i];
+          u[4 * i] = w.x;
+          u[4 * i + 1] = w.y;
+          u[4 * i + 2] = w.z;
+          u[4 * i + 3] = w.w;
+        }
+#pragma unroll
+        for (int g = 0; g < SB / 4; ++g) {
+          uint64_t v[4];
+#pragma unroll
+          for (int e = 0; e < 4; ++e) {
+            const int i = 4 * g + e;  // symbol of the batch
+            if constexpr (sizeof(T) == 2) v[e] = (u[i >> 1] >> (16 * (i & 1))) & 0xffffu;
+            else if constexpr (sizeof(T) == 4) v[e] = u[i];
+            else v[e] = uint64_t(u[2 * i]) | (uint64_t(u[2 * i + 1]) << 32);
+          }
+          planes4(v, b * (SB / 4) + g);
+        }
+      }
+      return;
+    }
+  }
+#pragma unroll 1
+  for (int q = 0; q < 16; ++q) {
+    uint64_t v[4];
+    if (cnt >= 4 * q + 4) {
+      if constexpr (sizeof(T) == 1) {
+        const uint32_t x = *gmem(reinterpret_cast<const uint32_t*>(static_cast<const T*>(text) + e0 + 4 * q));
+        v[0] = x & 0xff; v[1] = (x >> 8) & 0xff; v[2] = (x >> 16) & 0xff; v[3] = x >> 24;
+      } else {
+#pragma unroll
+        for (int e = 0; e < 4; ++e) v[e] = tx[e0 + 4 * q + e];
+      }
+    } else {
+#pragma unroll
+      for (int e = 0; e < 4; ++e) v[e] = (4 * q + e < cnt) ? uint64_t(tx[e0 + 4 * q + e]) : 0;
+    }
+    planes4(v, q);
   }
 }
 
@@ -664,7 +701,8 @@ __device__ __forceinline__ uint64_t compress64h(uint64_t x, uint64_t m, const ui
 // With SC (a pass followed by a symbol reorder) the bucket counts of the
 // order after all G levels (2^G buckets, the next pass's level order) are
 // produced as well, as "sub-level G".
-template <int G, int SC>
+// WIDE: symbols of 2, 4 or 8 bytes; otherwise bytes.
+template <int G, int SC, bool WIDE>
 __global__ __launch_bounds__(64 * 16) void k_local_small(K1Params p) {
   constexpr int WAVES = 16;
   constexpr int NSUB = G > 1 ? G - 1 : 1;  // sub-levels 1..G-1 (bits assembled)
@@ -712,7 +750,7 @@ __global__ __launch_bounds__(64 * 16) void k_local_small(K1Params p) {
     for (int j = 0; j < G; ++j) zsum[j] = 0;
     zchunks = 0;
   };
-  const bool fast_mode = p.width == 1;
+  const bool fast_mode = !WIDE;
   auto full_chunk = [&](const ChunkView& v) {
     return v.base + kChunkSyms <= v.n && (reinterpret_cast<uintptr_t>(v.text) & 15) == 0;
   };
@@ -757,11 +795,14 @@ __global__ __launch_bounds__(64 * 16) void k_local_small(K1Params p) {
     } else {
       if (cnt > 0) {
         const uint64_t e0 = cv.base + 64ull * lane;
-        switch (p.width) {
-          case 1: load_planes_text<G, uint8_t>(p, cv.text, e0, cnt, P0, my_or); break;
-          case 2: load_planes_text<G, uint16_t>(p, cv.text, e0, cnt, P0, my_or); break;
-          case 4: load_planes_text<G, uint32_t>(p, cv.text, e0, cnt, P0, my_or); break;
-          default: load_planes_text<G, uint64_t>(p, cv.text, e0, cnt, P0, my_or); break;
+        if constexpr (!WIDE) {
+          load_planes_text<G, uint8_t>(p, cv.text, e0, cnt, P0, my_or);
+        } else {
+          switch (p.width) {
+            case 2: load_planes_text<G, uint16_t>(p, cv.text, e0, cnt, P0, my_or); break;
+            case 4: load_planes_text<G, uint32_t>(p, cv.text, e0, cnt, P0, my_or); break;
+            default: load_planes_text<G, uint64_t>(p, cv.text, e0, cnt, P0, my_or); break;
+          }
         }
       }
       if (cn < p.nchunks && fast_mode && full_chunk(nv)) load_chunk(nv, pf);
@@ -887,7 +928,7 @@ constexpr int kSuper = 16;                                  // chunks per super-
 constexpr uint64_t kSuperSyms = uint64_t(kSuper) * kChunkSyms;
 constexpr uint32_t kSuperWords = kSuper * kChunkWords;      // words per sub-level
 
-template <int G>
+template <int G, bool WIDE>
 __global__ __launch_bounds__(64 * kSuper) void k_local_super(K1Params p) {
   static_assert(G >= 2 && G <= 4, "two to four levels");
   constexpr int WAVES = kSuper;
@@ -913,7 +954,7 @@ __global__ __launch_bounds__(64 * kSuper) void k_local_super(K1Params p) {
   for (int j = 0; j < G; ++j) zsum[j] = 0;
   uint64_t my_or = 0;
   const uint64_t nsuper = (p.nchunks + WAVES - 1) / WAVES;
-  const bool fast_mode = p.width == 1 && (reinterpret_cast<uintptr_t>(p.text) & 15) == 0;
+  const bool fast_mode = !WIDE && (reinterpret_cast<uintptr_t>(p.text) & 15) == 0;
   auto full_chunk = [&](uint64_t c) { return fast_mode && c < p.nchunks && (c + 1) * kChunkSyms <= p.n; };
   auto load_chunk = [&](uint64_t c, uint4* d) {
     const u32x4_t* src = reinterpret_cast<const u32x4_t*>(static_cast<const uint8_t*>(p.text) + c * kChunkSyms +
@@ -945,11 +986,14 @@ __global__ __launch_bounds__(64 * kSuper) void k_local_super(K1Params p) {
     } else {
       if (cnt > 0) {
         const uint64_t e0 = c * kChunkSyms + 64ull * lane;
-        switch (p.width) {
-          case 1: load_planes_text<G, uint8_t>(p, p.text, e0, cnt, P0, my_or); break;
-          case 2: load_planes_text<G, uint16_t>(p, p.text, e0, cnt, P0, my_or); break;
-          case 4: load_planes_text<G, uint32_t>(p, p.text, e0, cnt, P0, my_or); break;
-          default: load_planes_text<G, uint64_t>(p, p.text, e0, cnt, P0, my_or); break;
+        if constexpr (!WIDE) {
+          load_planes_text<G, uint8_t>(p, p.text, e0, cnt, P0, my_or);
+        } else {
+          switch (p.width) {
+            case 2: load_planes_text<G, uint16_t>(p, p.text, e0, cnt, P0, my_or); break;
+            case 4: load_planes_text<G, uint32_t>(p, p.text, e0, cnt, P0, my_or); break;
+            default: load_planes_text<G, uint64_t>(p, p.text, e0, cnt, P0, my_or); break;
+          }
         }
       }
       if (full_chunk(cn)) load_chunk(cn, pf);
@@ -1070,53 +1114,76 @@ __global__ __launch_bounds__(64 * kSuper) void k_local_super(K1Params p) {
 // symbols in the order of the level below: stable by bucket q = (bit of the
 // pass's last level ... bit of its first level), q ascending.  Only the bits
 // below the pass are kept, in the narrowest type that holds them.  One wave
-// per chunk, 64 symbols per step (one per lane, coalesced): the step's bucket
-// masks come from G ballots; a lane's destination is its bucket's running
-// position (held by lane q for bucket q) plus its rank among the step's lanes
-// of that bucket.  No LDS, one pass over the chunk.
+// per chunk, 64 symbols per step (one per lane, read coalesced): the step's
+// bucket masks come from G ballots; a lane's destination is its bucket's
+// running position (held by lane q for bucket q) plus its rank among the
+// step's lanes of that bucket.
 struct ScatterParams {
   const void* in;
   void* out;
   uint64_t n;
   uint32_t sh;               // the G-bit field of the pass's levels starts at bit sh
   const uint64_t* dstpos;    // [c * 2^G + q]: global position of (chunk, bucket)
+  const uint32_t* counts;    // [c * nslot + slot0 + q]: the chunk's bucket sizes
+  uint32_t nslot, slot0;
   uint64_t nchunks;
 };
+
+template <typename TO>
+constexpr int scatter_waves() {
+  return 4;
+}
+
+// Rank of this lane among the lanes whose bit is set in m (m's bits below
+// the lane): v_mbcnt.
+__device__ __forceinline__ uint32_t mbcnt64(uint64_t m) {
+  return __builtin_amdgcn_mbcnt_hi(uint32_t(m >> 32), __builtin_amdgcn_mbcnt_lo(uint32_t(m), 0u));
+}
 
 template <typename TI, typename TO, int G>
 __global__ __launch_bounds__(256) void k_scatter(ScatterParams p) {
   constexpr int NQ = 1 << G;
+  __shared__ uint64_t runs[4][NQ];  // per wave: running position of every bucket
   const int lane = threadIdx.x & 63;
-  const uint64_t c = uint64_t(blockIdx.x) * 4 + (threadIdx.x >> 6);
+  const int wid = threadIdx.x >> 6;
+  const uint64_t c = uint64_t(blockIdx.x) * 4 + wid;
   if (c >= p.nchunks) return;  // whole waves; no barrier below
+  uint64_t* run = runs[wid];
   const auto in = gmem(static_cast<const TI*>(p.in) + c * kChunkSyms);
   const auto out = gmem(static_cast<TO*>(p.out));
   const uint32_t E = uint32_t(min<uint64_t>(kChunkSyms, p.n - c * kChunkSyms));
   const uint64_t low = p.sh >= 64 ? ~0ull : ((1ull << p.sh) - 1);  // bits kept
-  uint64_t run = lane < NQ ? p.dstpos[c * NQ + lane] : 0;           // lane q: bucket q
-  const uint64_t lt = lane ? (~0ull >> (64 - lane)) : 0ull;
-  for (uint32_t t0 = 0; t0 < E; t0 += 256) {
-    TI x[4];
+  if (lane < NQ) run[lane] = p.dstpos[c * NQ + lane];
+  wave_sync();
+  // 16 steps of 64 symbols per round trip: their loads are issued together
+  constexpr int U = 16;
+  for (uint32_t t0 = 0; t0 < E; t0 += 64 * U) {
+    TI x[U];
 #pragma unroll
-    for (int u = 0; u < 4; ++u) {
+    for (int u = 0; u < U; ++u) {
       const uint32_t i = t0 + 64 * u + lane;
       x[u] = i < E ? in[i] : TI(0);
     }
 #pragma unroll
-    for (int u = 0; u < 4; ++u) {
+    for (int u = 0; u < U; ++u) {
       const bool valid = t0 + 64 * u + lane < E;
       const uint32_t f = uint32_t(uint64_t(x[u]) >> p.sh) & uint32_t(NQ - 1);
       const uint32_t q = __builtin_bitreverse32(f) >> (32 - G);
-      uint64_t mine = __ballot(valid), own = mine;
+      // lanes of this lane's bucket
+      uint64_t mine = __ballot(valid);
 #pragma unroll
       for (int b = 0; b < G; ++b) {
         const uint64_t B = __ballot(valid && ((q >> b) & 1));
         mine &= ((q >> b) & 1) ? B : ~B;
-        own &= ((lane >> b) & 1) ? B : ~B;
       }
-      const uint64_t base = __shfl(run, int(q), 64);
-      if (valid) out[base + __popcll(mine & lt)] = TO(uint64_t(x[u]) & low);
-      run += __popcll(own);
+      const uint32_t rank = mbcnt64(mine);
+      const uint64_t base = run[q];
+      wave_sync();  // every lane has read its bucket's position
+      if (valid) {
+        out[base + rank] = TO(uint64_t(x[u]) & low);
+        if (rank + 1 == uint32_t(__popcll(mine))) run[q] = base + rank + 1;  // the bucket's last lane
+      }
+      wave_sync();
     }
   }
 }
@@ -2132,7 +2199,10 @@ template <int G>
 void launch_super(pwmhip_ctx* ctx, const K1Params& prm) {
   const uint64_t nsuper = (prm.nchunks + kSuper - 1) / kSuper;
   const uint64_t grid = std::min<uint64_t>(nsuper, uint64_t(ctx->num_cus));
-  hipLaunchKernelGGL((k_local_super<G>), dim3(uint32_t(grid)), dim3(64 * kSuper), 0, ctx->stream, prm);
+  if (prm.width == 1)
+    hipLaunchKernelGGL((k_local_super<G, false>), dim3(uint32_t(grid)), dim3(64 * kSuper), 0, ctx->stream, prm);
+  else
+    hipLaunchKernelGGL((k_local_super<G, true>), dim3(uint32_t(grid)), dim3(64 * kSuper), 0, ctx->stream, prm);
   PWM_HIP_TRY(hipGetLastError());
 }
 
@@ -2147,7 +2217,10 @@ void launch_k1(pwmhip_ctx* ctx, const K1Params& prm) {
 template <int G, int SC>
 void launch_small(pwmhip_ctx* ctx, const K1Params& prm) {
   const uint64_t grid = std::min<uint64_t>((prm.nchunks + 15) / 16, uint64_t(ctx->num_cus));
-  hipLaunchKernelGGL((k_local_small<G, SC>), dim3(uint32_t(grid)), dim3(64 * 16), 0, ctx->stream, prm);
+  if (prm.width == 1)
+    hipLaunchKernelGGL((k_local_small<G, SC, false>), dim3(uint32_t(grid)), dim3(64 * 16), 0, ctx->stream, prm);
+  else
+    hipLaunchKernelGGL((k_local_small<G, SC, true>), dim3(uint32_t(grid)), dim3(64 * 16), 0, ctx->stream, prm);
   PWM_HIP_TRY(hipGetLastError());
 }
 
@@ -2232,14 +2305,20 @@ uint32_t gather_windows(uint64_t W, uint32_t num_cus) {
   return uint32_t((nwin + kGroup - 1) / kGroup * kGroup);
 }
 
+template <typename TI, typename TO, int G>
+void launch_scatter_k(hipStream_t s, const ScatterParams& sp) {
+  constexpr int WB = scatter_waves<TO>();
+  const dim3 grid(uint32_t((sp.nchunks + WB - 1) / WB)), block(64 * WB);
+  hipLaunchKernelGGL((k_scatter<TI, TO, G>), grid, block, 0, s, sp);
+}
+
 template <typename TI, int G>
 void launch_scatter_o(hipStream_t s, uint32_t wout, const ScatterParams& sp) {
-  const dim3 grid(uint32_t((sp.nchunks + 3) / 4)), block(256);
   switch (wout) {
-    case 1: hipLaunchKernelGGL((k_scatter<TI, uint8_t, G>), grid, block, 0, s, sp); break;
-    case 2: hipLaunchKernelGGL((k_scatter<TI, uint16_t, G>), grid, block, 0, s, sp); break;
-    case 4: hipLaunchKernelGGL((k_scatter<TI, uint32_t, G>), grid, block, 0, s, sp); break;
-    default: hipLaunchKernelGGL((k_scatter<TI, uint64_t, G>), grid, block, 0, s, sp); break;
+    case 1: launch_scatter_k<TI, uint8_t, G>(s, sp); break;
+    case 2: launch_scatter_k<TI, uint16_t, G>(s, sp); break;
+    case 4: launch_scatter_k<TI, uint32_t, G>(s, sp); break;
+    default: launch_scatter_k<TI, uint64_t, G>(s, sp); break;
   }
 }
 
@@ -2385,6 +2464,9 @@ void build_passes_device(pwmhip_ctx* ctx, const void* d_text, uint64_t n, uint32
         sp.n = n;
         sp.sh = R - g;
         sp.dstpos = U64(11);
+        sp.counts = counts;
+        sp.nslot = nslot;
+        sp.slot0 = (1u << g) - 2u;
         sp.nchunks = nchunks;
         const uint32_t wout = width_for_bits(R - g);
         launch_scatter(s, cw, wout, g, sp);
