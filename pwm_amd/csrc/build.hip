@@ -1109,6 +1109,173 @@ __global__ __launch_bounds__(64 * kSuper) void k_local_super(K1Params p) {
   }
 }
 
+// ---- Huffman-shaped local build over super-chunks -------------------------
+// Level l of a Huffman-shaped structure holds the occurrences whose code is
+// longer than l, grouped by node (their l-bit code prefix), text order
+// inside a node.  With every symbol mapped to its code left-aligned in G
+// bits, a node of level j is a bucket of the plain construction's sub-level
+// j (bit i of the bucket index = code bit i), and whether a bucket is a node
+// depends on the prefix alone (codes are prefix-free), so the plain
+// super-chunk assembly applies with the buckets that are no node left out.
+// Sub-level buckets are laid out in slot order (the stitch's order).
+template <int G>
+__global__ __launch_bounds__(64 * kSuper) void k_huff_super(HuffSuperParams hp, const uint8_t* lut_src) {
+  static_assert(G >= 2 && G <= 4, "two to four levels");
+  constexpr int WAVES = kSuper;
+  constexpr int NSUB = G - 1;
+  constexpr int JC = G - 1;
+  constexpr int NQ = 1 << JC;
+  constexpr int NS = (NQ + 1) / 2;
+  __shared__ __align__(16) uint8_t lut[65536];
+  __shared__ uint64_t img[NSUB][kSuperWords];
+  __shared__ uint32_t wt[WAVES][NQ];
+  __shared__ uint8_t cmap[256];
+  for (uint32_t i = threadIdx.x; i < NSUB * kSuperWords; i += 64 * WAVES) (&img[0][0])[i] = 0;
+  if (threadIdx.x < 256) cmap[threadIdx.x] = hp.code[threadIdx.x];
+  const int lane = threadIdx.x & 63;
+  const int wid = threadIdx.x >> 6;
+  const int uwid = __builtin_amdgcn_readfirstlane(wid);
+  load_lut<WAVES>(lut, lut_src);  // (ends with a barrier)
+  K1Params kp{};
+  kp.L = G;
+  kp.l0 = 0;
+  kp.np = G;
+  // codes of symbols 0..7 as two permute tables
+  const uint32_t tlo = uint32_t(cmap[0]) | (uint32_t(cmap[1]) << 8) | (uint32_t(cmap[2]) << 16) |
+                       (uint32_t(cmap[3]) << 24);
+  const uint32_t thi = uint32_t(cmap[4]) | (uint32_t(cmap[5]) << 8) | (uint32_t(cmap[6]) << 16) |
+                       (uint32_t(cmap[7]) << 24);
+  auto map4 = [&](uint32_t x) -> uint32_t {
+    if (hp.small_alpha) return __builtin_amdgcn_perm(thi, tlo, x);
+    return uint32_t(cmap[x & 0xff]) | (uint32_t(cmap[(x >> 8) & 0xff]) << 8) |
+           (uint32_t(cmap[(x >> 16) & 0xff]) << 16) | (uint32_t(cmap[x >> 24]) << 24);
+  };
+  const uint64_t nsuper = (hp.nchunks + WAVES - 1) / WAVES;
+  const bool aligned = (reinterpret_cast<uintptr_t>(hp.text) & 15) == 0;
+  auto full_chunk = [&](uint64_t c) { return aligned && c < hp.nchunks && (c + 1) * kChunkSyms <= hp.n; };
+  auto load_chunk = [&](uint64_t c, uint4* d) {
+    const auto src = gmem(reinterpret_cast<const uint4*>(hp.text + c * kChunkSyms + 64ull * lane));
+#pragma unroll
+    for (int i = 0; i < 4; ++i) d[i] = src[i];
+  };
+  uint4 pf[4];
+  uint64_t sc = blockIdx.x;
+  if (sc < nsuper && full_chunk(sc * WAVES + wid)) load_chunk(sc * WAVES + wid, pf);
+
+  for (; sc < nsuper; sc += gridDim.x) {
+    const uint64_t c = sc * WAVES + wid;
+    const uint64_t cn = (sc + gridDim.x) * WAVES + wid;
+    const bool have = c < hp.nchunks;
+    const uint32_t Ec = have ? uint32_t(min<uint64_t>(kChunkSyms, hp.n - c * kChunkSyms)) : 0u;
+    const uint32_t nw = (Ec + 63) >> 6;
+    const int cnt = max(0, min(64, int(Ec) - 64 * lane));
+
+    uint64_t P0[G];
+#pragma unroll
+    for (int k = 0; k < G; ++k) P0[k] = 0;
+    if (full_chunk(c)) {
+      uint4 m[4];
+#pragma unroll
+      for (int i = 0; i < 4; ++i) m[i] = make_uint4(map4(pf[i].x), map4(pf[i].y), map4(pf[i].z), map4(pf[i].w));
+      uint64_t unused = 0;
+      planes_from_bytes<G>(kp, m, P0, unused);
+      if (full_chunk(cn)) load_chunk(cn, pf);  // prefetch
+    } else {
+      const auto tx = gmem(hp.text);
+      for (int i = 0; i < cnt; ++i) {
+        const uint32_t a = cmap[tx[c * kChunkSyms + 64ull * lane + i]];
+#pragma unroll
+        for (int k = 0; k < G; ++k) P0[k] |= uint64_t((a >> (G - 1 - k)) & 1u) << i;
+      }
+      if (full_chunk(cn)) load_chunk(cn, pf);
+    }
+    const uint64_t vm = cnt >= 64 ? ~0ull : (cnt <= 0 ? 0ull : ((1ull << cnt) - 1));
+    if (lane < int(nw)) st_stream(hp.level0 + c * kChunkWords + lane, brev64(P0[0]));
+
+    uint64_t M[NQ];
+    M[0] = vm;
+#pragma unroll
+    for (int i = 0; i < JC; ++i) {
+#pragma unroll
+      for (int q = (1 << i) - 1; q >= 0; --q) {
+        M[q + (1 << i)] = M[q] & P0[i];
+        M[q] = M[q] & ~P0[i];
+      }
+    }
+    uint32_t f[2 * NS];
+#pragma unroll
+    for (int q = 0; q < NQ; ++q) f[q] = __popcll(M[q]);
+    if (NQ < 2 * NS) f[2 * NS - 1] = 0;
+    uint32_t ex[2 * NS], tot[2 * NS];
+#pragma unroll
+    for (int s2 = 0; s2 < NS; ++s2) {
+      const uint32_t v = f[2 * s2] | (f[2 * s2 + 1] << 16);
+      const uint32_t inc = wave_incl_scan(v);
+      const uint32_t t = __builtin_amdgcn_readlane(inc, 63);
+      const uint32_t e = inc - v;
+      ex[2 * s2] = e & 0xffffu;
+      ex[2 * s2 + 1] = e >> 16;
+      tot[2 * s2] = t & 0xffffu;
+      tot[2 * s2 + 1] = t >> 16;
+    }
+    if (lane < NQ) {
+      uint32_t v = 0;
+#pragma unroll
+      for (int q = 0; q < NQ; ++q) v = lane == q ? tot[q] : v;
+      wt[wid][lane] = v;
+    }
+    lds_sync();
+
+    uint32_t stot[NQ], spre[NQ];
+#pragma unroll
+    for (int q = 0; q < NQ; ++q) {
+      const uint32_t v = lane < WAVES ? wt[lane][q] : 0u;
+      const uint32_t inc = wave_incl_scan(v);
+      stot[q] = __builtin_amdgcn_readlane(inc, WAVES - 1);
+      spre[q] = __builtin_amdgcn_readlane(inc - v, uwid);
+    }
+#pragma unroll
+    for (int j = 1; j <= JC; ++j) {
+      const int nb = 1 << j;
+      uint32_t T[1 << JC];
+#pragma unroll
+      for (int q = 0; q < nb; ++q) {
+        T[q] = 0;
+#pragma unroll
+        for (int r = q; r < NQ; r += nb) T[q] += stot[r];
+      }
+#pragma unroll
+      for (int q = 0; q < nb; ++q) {
+        const uint32_t rk = hp.rank[j][q];
+        if (rk == 0xffu) continue;  // no node
+        uint32_t start = 0, E = 0;
+#pragma unroll
+        for (int q2 = 0; q2 < nb; ++q2) {
+          const uint32_t r2 = hp.rank[j][q2];
+          if (r2 != 0xffu && r2 < rk) start += T[q2];
+        }
+#pragma unroll
+        for (int r = q; r < NQ; r += nb) E += spre[r] + ex[r];
+        uint64_t mq = vm;
+#pragma unroll
+        for (int i = 0; i < j; ++i) mq &= ((q >> i) & 1) ? P0[i] : ~P0[i];
+        or_bits(img[j - 1], start + E, compress64h(P0[j], mq, lut), __popcll(mq));
+        if (uwid == 0 && lane == 0) gmem(hp.counts)[sc * hp.nslot + hp.slot0[j] + rk] = T[q];
+      }
+    }
+    lds_sync();
+
+    const uint64_t s0 = sc * kSuperSyms;
+    const uint32_t nws = uint32_t((min<uint64_t>(kSuperSyms, hp.n - s0) + 63) >> 6);
+#pragma unroll
+    for (int j = 0; j < NSUB; ++j) {
+      if (threadIdx.x < nws)
+        st_stream(hp.temp + j * hp.temp_stride + sc * kSuperWords + threadIdx.x, brev64(img[j][threadIdx.x]));
+      img[j][threadIdx.x] = 0;
+    }
+  }
+}
+
 // ---- symbol reorder between passes ------------------------------------------
 // After a pass over the top G levels of its symbols, the next pass reads the
 // symbols in the order of the level below: stable by bucket q = (bit of the
@@ -2205,6 +2372,24 @@ void launch_super(pwmhip_ctx* ctx, const K1Params& prm) {
     hipLaunchKernelGGL((k_local_super<G, true>), dim3(uint32_t(grid)), dim3(64 * kSuper), 0, ctx->stream, prm);
   PWM_HIP_TRY(hipGetLastError());
 }
+
+}  // namespace
+
+void launch_huff_super(pwmhip_ctx* ctx, const HuffSuperParams& hp) {
+  static_assert(kSuperChunkSyms == kSuperSyms, "one super-chunk size");
+  const uint64_t nsuper = (hp.nchunks + kSuper - 1) / kSuper;
+  const uint32_t grid = uint32_t(std::min<uint64_t>(nsuper, uint64_t(ctx->num_cus)));
+  const uint8_t* lut = compress_table(ctx);
+  switch (hp.levels) {
+    case 2: hipLaunchKernelGGL(k_huff_super<2>, dim3(grid), dim3(64 * kSuper), 0, ctx->stream, hp, lut); break;
+    case 3: hipLaunchKernelGGL(k_huff_super<3>, dim3(grid), dim3(64 * kSuper), 0, ctx->stream, hp, lut); break;
+    case 4: hipLaunchKernelGGL(k_huff_super<4>, dim3(grid), dim3(64 * kSuper), 0, ctx->stream, hp, lut); break;
+    default: throw status_error{PWMHIP_ERR_INVALID, "Huffman super-chunk build: 2..4 levels"};
+  }
+  PWM_HIP_TRY(hipGetLastError());
+}
+
+namespace {
 
 template <int NP>
 void launch_k1(pwmhip_ctx* ctx, const K1Params& prm) {

@@ -22,6 +22,7 @@
 #include "pwm_internal.hpp"
 
 #include <algorithm>
+#include <cstdlib>
 #include <cstring>
 #include <functional>
 #include <numeric>
@@ -63,6 +64,79 @@ __global__ void k_hist(const T* text, uint64_t n, uint32_t nsym, unsigned long l
   if (small)
     for (uint32_t i = threadIdx.x; i < 256; i += blockDim.x)
       if (lh[i]) atomicAdd(&hist[i], (unsigned long long)lh[i]);
+}
+
+__device__ __forceinline__ uint32_t mbcnt64(uint64_t m) {
+  return __builtin_amdgcn_mbcnt_hi(uint32_t(m >> 32), __builtin_amdgcn_mbcnt_lo(uint32_t(m), 0u));
+}
+
+// Byte texts: counts of 8 candidate values (the most frequent in a sample)
+// in registers, 4 bytes at a time: byte k of d = x ^ (v * 0x01010101) is 0
+// exactly when bit 7 of byte k of ~(((d & 0x7f..) + 0x7f..) | d | 0x7f..)
+// is set.  Bytes equal to no candidate go to LDS atomics.  16 bytes per
+// lane per step.
+struct HistCand {
+  uint32_t rep[8];  // candidate value repeated in 4 bytes
+  uint32_t val[8];
+  uint32_t used;    // number of candidates
+};
+
+__device__ __forceinline__ uint32_t zero_bytes(uint32_t d) {
+  return ~(((d & 0x7f7f7f7fu) + 0x7f7f7f7fu) | d | 0x7f7f7f7fu);
+}
+
+__global__ __launch_bounds__(256) void k_hist_bytes(const uint8_t* text, uint64_t n, HistCand hc,
+                                                    unsigned long long* hist) {
+  __shared__ unsigned int lh[256];
+  for (uint32_t i = threadIdx.x; i < 256; i += blockDim.x) lh[i] = 0;
+  __syncthreads();
+  uint32_t cnt[8] = {0, 0, 0, 0, 0, 0, 0, 0};
+  const bool aligned = (reinterpret_cast<uintptr_t>(text) & 15) == 0;
+  const uint64_t tid = uint64_t(blockIdx.x) * blockDim.x + threadIdx.x;
+  const uint64_t nthr = uint64_t(gridDim.x) * blockDim.x;
+  for (uint64_t i0 = tid * 16; i0 < n; i0 += nthr * 16) {
+    uint32_t w[4] = {0, 0, 0, 0}, vmask[4] = {0, 0, 0, 0};
+    if (i0 + 16 <= n && aligned) {
+      const uint4 v = *reinterpret_cast<const uint4*>(text + i0);
+      w[0] = v.x; w[1] = v.y; w[2] = v.z; w[3] = v.w;
+      vmask[0] = vmask[1] = vmask[2] = vmask[3] = 0x80808080u;
+    } else {
+      const uint32_t nv = uint32_t(min<uint64_t>(16, n - i0));
+      for (uint32_t k = 0; k < nv; ++k) {
+        w[k >> 2] |= uint32_t(text[i0 + k]) << (8 * (k & 3));
+        vmask[k >> 2] |= 0x80u << (8 * (k & 3));
+      }
+    }
+#pragma unroll
+    for (int q = 0; q < 4; ++q) {
+      uint32_t hit = 0;
+#pragma unroll
+      for (int k = 0; k < 8; ++k) {
+        if (uint32_t(k) < hc.used) {
+          const uint32_t z = zero_bytes(w[q] ^ hc.rep[k]) & vmask[q];
+          cnt[k] += __builtin_popcount(z);
+          hit |= z;
+        }
+      }
+      uint32_t miss = vmask[q] & ~hit;
+      while (miss) {
+        const uint32_t bit = __builtin_ctz(miss);
+        atomicAdd(&lh[(w[q] >> (bit - 7)) & 0xffu], 1u);
+        miss &= miss - 1;
+      }
+    }
+  }
+  // candidate counts: wave sums, one atomic per wave and candidate
+#pragma unroll
+  for (int k = 0; k < 8; ++k) {
+    uint32_t v = cnt[k];
+#pragma unroll
+    for (int d = 32; d >= 1; d >>= 1) v += __shfl_xor(v, d, 64);
+    if ((threadIdx.x & 63) == 0 && uint32_t(k) < hc.used && v) atomicAdd(&lh[hc.val[k]], v);
+  }
+  __syncthreads();
+  for (uint32_t i = threadIdx.x; i < 256; i += blockDim.x)
+    if (lh[i]) atomicAdd(&hist[i], (unsigned long long)lh[i]);
 }
 
 struct HParams {
@@ -205,9 +279,11 @@ struct HStitch {
   const uint64_t* level_bits;  // [l]
   uint32_t nslot, levels;
   int all_atomic;
+  uint64_t unit_syms;          // symbols per run unit (chunk or super-chunk)
+  uint32_t temp_l0;            // level l of temp at temp + (l - temp_l0) * temp_stride
 };
 
-// One block per chunk, one thread per (run, destination word), runs looped.
+// One block per unit, one thread per (run, destination word), runs looped.
 __global__ __launch_bounds__(256) void k_huff_stitch(HStitch p) {
   const uint64_t c = blockIdx.x;
   for (uint32_t l = 1; l < p.levels; ++l) {
@@ -215,13 +291,13 @@ __global__ __launch_bounds__(256) void k_huff_stitch(HStitch p) {
     for (uint32_t s = p.level_slot0[l]; s < p.level_slot0[l + 1]; ++s) {
       const uint64_t len = p.counts[c * p.nslot + s];
       if (!len) continue;
-      const uint64_t src = c * kHChunk + loc;
+      const uint64_t src = c * p.unit_syms + loc;
       loc += len;
       const uint64_t dbit = p.gstart[s] + p.pref[c * p.nslot + s];
       const uint64_t w0 = dbit >> 6, w1 = (dbit + len - 1) >> 6;
       const uint64_t nbits = p.level_bits[l];
       uint64_t* dst = p.dst[l];
-      const uint64_t* sa = p.temp + l * p.temp_stride;
+      const uint64_t* sa = p.temp + (l - p.temp_l0) * p.temp_stride;
       for (uint64_t w = w0 + threadIdx.x; w <= w1; w += blockDim.x) {
         const uint64_t ws = w << 6;
         const uint64_t a = max(ws, dbit), b = min(ws + 64, dbit + len);
@@ -336,6 +412,121 @@ inline uint32_t rev(uint64_t x, uint64_t bits) {
 
 }  // namespace
 
+// PWMHIP_HUFF_CHUNKS=1: the general per-chunk Huffman kernels for every input.
+bool huffman_super_disabled() {
+  static const bool off = [] {
+    const char* e = std::getenv("PWMHIP_HUFF_CHUNKS");
+    return e && e[0] == '1';
+  }();
+  return off;
+}
+
+// Byte texts, codes of 2..4 bits, node ranges that do not overlap: local
+// build over super-chunks (level 0 in place), scan of the node counts over
+// super-chunks, run stitch into the cleared levels 1..L-1.
+void build_huffman_super(pwmhip_ctx* ctx, const uint8_t* d_text, uint64_t n, const code_table& ct, uint64_t L,
+                         uint64_t maxsym, const std::vector<uint64_t>& slot_prefix,
+                         const std::vector<uint32_t>& slot_level, const std::vector<uint32_t>& level_slot0,
+                         const std::vector<uint64_t>& start, uint32_t nslot, const std::vector<uint64_t>& bits,
+                         pwmhip_ws* ws) {
+  hipStream_t st = ctx->stream;
+  const uint64_t nchunks = (n + kChunkSyms - 1) / kChunkSyms;
+  const uint64_t units = (nchunks + 15) / 16;
+  const uint64_t tstride = units * (kSuperChunkSyms / 64) + 64;
+  const uint32_t ns = std::max<uint32_t>(nslot, 1);
+  size_t need = 0;
+  auto add = [&](size_t b) { need += (b + 255) & ~size_t(255); };
+  add(256); add((L - 1) * tstride * 8); add(units * ns * 4); add(units * ns * 8); add(ns * 8 + 8);
+  add(ns * 8 + 8); add(L * 8); add(L * 8); add(scan_scratch_words(units, nslot) * 8); add((L + 1) * 4);
+  add(ns * 4 + 4);
+  ctx->scratch.reset();
+  ctx->scratch.ensure(need);
+  arena& A = ctx->scratch;
+  uint8_t* d_code = A.take<uint8_t>(256);
+  uint64_t* d_temp = A.take<uint64_t>((L - 1) * tstride);
+  uint32_t* d_counts = A.take<uint32_t>(units * ns);
+  uint64_t* d_pref = A.take<uint64_t>(units * ns);
+  uint64_t* d_tot = A.take<uint64_t>(ns + 1);
+  uint64_t* d_start = A.take<uint64_t>(ns + 1);
+  uint64_t** d_dst = A.take<uint64_t*>(L);
+  uint64_t* d_lbits = A.take<uint64_t>(L);
+  uint64_t* d_tiles = A.take<uint64_t>(scan_scratch_words(units, nslot));
+  uint32_t* d_level_slot0 = A.take<uint32_t>(L + 1);
+  uint32_t* d_slot_level = A.take<uint32_t>(ns + 1);
+
+  std::vector<uint8_t> code(256, 0);
+  for (uint64_t s = 0; s < ct.len.size() && s < 256; ++s)
+    if (ct.len[s]) code[s] = uint8_t(ct.word[s] << (L - ct.len[s]));
+  HuffSuperParams hp{};
+  hp.text = d_text;
+  hp.n = n;
+  hp.nchunks = nchunks;
+  hp.levels = uint32_t(L);
+  hp.code = d_code;
+  hp.small_alpha = maxsym < 8 ? 1u : 0u;
+  hp.level0 = ws->d_words + ws->level_offset[0];
+  hp.temp = d_temp;
+  hp.temp_stride = tstride;
+  hp.counts = d_counts;
+  hp.nslot = nslot;
+  std::memset(hp.rank, 0xff, sizeof(hp.rank));
+  for (uint64_t j = 1; j < L; ++j) {
+    hp.slot0[j] = level_slot0[j];
+    for (uint32_t sl = level_slot0[j]; sl < level_slot0[j + 1]; ++sl)
+      hp.rank[j][rev(slot_prefix[sl], j)] = uint8_t(sl - level_slot0[j]);
+  }
+  std::vector<uint64_t*> dst(L);
+  for (uint64_t l = 0; l < L; ++l) dst[l] = ws->d_words + ws->level_offset[l];
+  PWM_HIP_TRY(hipMemcpyAsync(d_code, code.data(), 256, hipMemcpyHostToDevice, st));
+  if (nslot) {
+    PWM_HIP_TRY(hipMemcpyAsync(d_start, start.data(), nslot * 8, hipMemcpyHostToDevice, st));
+    PWM_HIP_TRY(hipMemcpyAsync(d_slot_level, slot_level.data(), nslot * 4, hipMemcpyHostToDevice, st));
+  }
+  PWM_HIP_TRY(hipMemcpyAsync(d_level_slot0, level_slot0.data(), (L + 1) * 4, hipMemcpyHostToDevice, st));
+  PWM_HIP_TRY(hipMemcpyAsync(d_dst, dst.data(), L * sizeof(uint64_t*), hipMemcpyHostToDevice, st));
+  PWM_HIP_TRY(hipMemcpyAsync(d_lbits, bits.data(), L * 8, hipMemcpyHostToDevice, st));
+  // runs of levels >= 1 meet inside words (and huff_pc's node borders can
+  // leave gaps): the levels start cleared
+  for (uint64_t l = 1; l < L; ++l)
+    PWM_HIP_TRY(hipMemsetAsync(dst[l], 0, words_for(bits[l]) * 8, st));
+
+  PWM_HIP_TRY(hipEventRecord(ctx->ev[0], st));
+  launch_huff_super(ctx, hp);
+  PWM_HIP_TRY(hipEventRecord(ctx->ev[1], st));
+  if (nslot) {
+    launch_scan_slots(st, d_counts, d_pref, d_tot, units, nslot, d_tiles);
+    PWM_HIP_TRY(hipEventRecord(ctx->ev[2], st));
+    HStitch hs{};
+    hs.counts = d_counts;
+    hs.pref = d_pref;
+    hs.gstart = d_start;
+    hs.slot_level = d_slot_level;
+    hs.level_slot0 = d_level_slot0;
+    hs.temp = d_temp;
+    hs.temp_stride = tstride;
+    hs.dst = d_dst;
+    hs.level_bits = d_lbits;
+    hs.nslot = nslot;
+    hs.levels = uint32_t(L);
+    hs.all_atomic = 0;
+    hs.unit_syms = kSuperChunkSyms;
+    hs.temp_l0 = 1;
+    hipLaunchKernelGGL(k_huff_stitch, dim3(uint32_t(units)), dim3(256), 0, st, hs);
+    PWM_HIP_TRY(hipGetLastError());
+  } else {
+    PWM_HIP_TRY(hipEventRecord(ctx->ev[2], st));
+  }
+  PWM_HIP_TRY(hipEventRecord(ctx->ev[3], st));
+  PWM_HIP_TRY(hipStreamSynchronize(st));
+  float ms = 0;
+  for (int i = 0; i < 3; ++i) {
+    PWM_HIP_TRY(hipEventElapsedTime(&ms, ctx->ev[i], ctx->ev[i + 1]));
+    ctx->last_ms[i] = ms;
+  }
+  PWM_HIP_TRY(hipEventElapsedTime(&ms, ctx->ev[0], ctx->ev[3]));
+  ctx->last_ms[3] = ms;
+}
+
 void build_huffman_device(pwmhip_ctx* ctx, const void* d_text, uint64_t n, uint32_t width, bool tree,
                           bool pc_quirks, pwmhip_ws* ws) {
   hipStream_t st = ctx->stream;
@@ -356,7 +547,29 @@ void build_huffman_device(pwmhip_ctx* ctx, const void* d_text, uint64_t n, uint3
   unsigned long long* first_arg = width == 1 ? nullptr : d_first;
   if (n) {
     switch (width) {
-      case 1: hipLaunchKernelGGL(k_hist<uint8_t>, dim3(grid), dim3(256), 0, st, (const uint8_t*)d_text, n, nsym, d_hist, first_arg, d_max); break;
+      case 1: {
+        // candidates: the 8 most frequent values of a 1 MiB sample
+        const uint64_t ns = std::min<uint64_t>(n, 1 << 20);
+        const uint32_t gs = uint32_t(std::min<uint64_t>((ns + 1023) / 1024, 256));
+        hipLaunchKernelGGL(k_hist<uint8_t>, dim3(gs), dim3(256), 0, st, (const uint8_t*)d_text, ns, 256u, d_hist,
+                           (unsigned long long*)nullptr, d_max);
+        std::vector<uint64_t> sh(256);
+        PWM_HIP_TRY(hipMemcpyAsync(sh.data(), d_hist, 256 * 8, hipMemcpyDeviceToHost, st));
+        PWM_HIP_TRY(hipMemsetAsync(d_hist, 0, 256 * 8, st));
+        PWM_HIP_TRY(hipStreamSynchronize(st));
+        std::vector<uint32_t> ord(256);
+        std::iota(ord.begin(), ord.end(), 0u);
+        std::stable_sort(ord.begin(), ord.end(), [&](uint32_t x, uint32_t y) { return sh[x] > sh[y]; });
+        HistCand hc{};
+        for (uint32_t k = 0; k < 8; ++k) {
+          if (!sh[ord[k]]) break;
+          hc.val[k] = ord[k];
+          hc.rep[k] = ord[k] * 0x01010101u;
+          hc.used = k + 1;
+        }
+        hipLaunchKernelGGL(k_hist_bytes, dim3(grid), dim3(256), 0, st, (const uint8_t*)d_text, n, hc, d_hist);
+        break;
+      }
       case 2: hipLaunchKernelGGL(k_hist<uint16_t>, dim3(grid), dim3(256), 0, st, (const uint16_t*)d_text, n, nsym, d_hist, first_arg, d_max); break;
       case 4: hipLaunchKernelGGL(k_hist<uint32_t>, dim3(grid), dim3(256), 0, st, (const uint32_t*)d_text, n, nsym, d_hist, first_arg, d_max); break;
       default: hipLaunchKernelGGL(k_hist<uint64_t>, dim3(grid), dim3(256), 0, st, (const uint64_t*)d_text, n, nsym, d_hist, first_arg, d_max); break;
@@ -369,6 +582,11 @@ void build_huffman_device(pwmhip_ctx* ctx, const void* d_text, uint64_t n, uint3
   if (first_arg) PWM_HIP_TRY(hipMemcpyAsync(first.data(), d_first, nsym * 8, hipMemcpyDeviceToHost, st));
   PWM_HIP_TRY(hipMemcpyAsync(&maxsym, d_max, 8, hipMemcpyDeviceToHost, st));
   PWM_HIP_TRY(hipStreamSynchronize(st));
+  if (width == 1) {  // the byte histogram does not track the maximum
+    maxsym = 0;
+    for (uint64_t c = 0; c < 256; ++c)
+      if (hist[c]) maxsym = c;
+  }
   if (n && maxsym >= nsym) throw status_error{PWMHIP_ERR_UNSUPPORTED, "Huffman alphabet above 4096 symbols"};
 
   // 2. codes (histogram entry order as in util/histogram.hpp:39-76)
@@ -493,6 +711,14 @@ void build_huffman_device(pwmhip_ctx* ctx, const void* d_text, uint64_t n, uint3
   ws->n_zeros = tree ? 0 : L;
   for (uint64_t l = 0; l < L; ++l) ws->zeros[l] = tree ? 0 : zeros[l];
 
+  // 5a. byte texts with codes of 2..4 bits: the plain construction's
+  // super-chunk build with code-mapped symbols, then the run stitch
+  if (width == 1 && L >= 2 && L <= 4 && !overlap && !huffman_super_disabled()) {
+    build_huffman_super(ctx, static_cast<const uint8_t*>(d_text), n, ct, L, maxsym, slot_prefix, slot_level,
+                        level_slot0, start, nslot, bits, ws);
+    return;
+  }
+
   // 5. device tables and scratch
   const uint64_t nchunks = (n + kHChunk - 1) / kHChunk;
   const uint64_t tstride = level_stride(nchunks * kHChunk);
@@ -571,6 +797,8 @@ void build_huffman_device(pwmhip_ctx* ctx, const void* d_text, uint64_t n, uint3
     hs.nslot = nslot;
     hs.levels = uint32_t(L);
     hs.all_atomic = overlap ? 1 : 0;
+    hs.unit_syms = kHChunk;
+    hs.temp_l0 = 0;
     hipLaunchKernelGGL(k_huff_stitch, dim3(uint32_t(nchunks)), dim3(256), 0, st, hs);
     PWM_HIP_TRY(hipGetLastError());
   } else {

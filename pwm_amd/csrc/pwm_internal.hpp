@@ -145,6 +145,26 @@ void launch_scan_slots(hipStream_t s, const uint32_t* counts, uint64_t* pref, ui
                        uint64_t nchunks, uint32_t nslot, uint64_t* tile_scratch, bool indicator = false);
 uint64_t scan_scratch_words(uint64_t nchunks, uint32_t nslot);
 
+// Huffman-shaped local build for byte texts with codes of 2..4 bits
+// (build.hip): level 0 to its final place, sub-levels 1..levels-1 per
+// super-chunk (16 chunks) in slot order, counts per (super-chunk, slot).
+struct HuffSuperParams {
+  const uint8_t* text;
+  uint64_t n, nchunks;
+  uint32_t levels;            // 2..4
+  const uint8_t* code;        // device, 256 entries: code left-aligned in `levels` bits
+  uint32_t small_alpha;       // every symbol < 8 (byte permute instead of table reads)
+  uint64_t* level0;
+  uint64_t* temp;             // sub-level j at temp + (j - 1) * temp_stride
+  uint64_t temp_stride;
+  uint32_t* counts;           // [unit * nslot + slot]
+  uint32_t nslot;
+  uint32_t slot0[4];          // first slot of sub-level j
+  uint8_t rank[4][8];         // slot of bucket q of sub-level j, minus slot0[j]; 0xff: no node
+};
+void launch_huff_super(pwmhip_ctx* ctx, const HuffSuperParams& hp);
+constexpr uint64_t kSuperChunkSyms = uint64_t(16) * kChunkSyms;  // symbols per super-chunk
+
 // Huffman-shaped construction (huffman.hip).
 void build_huffman_device(pwmhip_ctx* ctx, const void* d_text, uint64_t n, uint32_t width,
                           bool tree, bool pc_quirks, pwmhip_ws* ws);

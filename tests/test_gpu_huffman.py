@@ -85,3 +85,25 @@ def test_huffman_pc_undefined_cases_are_rejected():
             assert e.value.status == pwm_amd.ERR_UNSUPPORTED
             rejected += 1
     assert rejected > 0
+
+
+@pytest.mark.gpu
+@pytest.mark.parametrize("values", [(0, 1, 2), (0, 1, 2, 3, 4), (3, 5, 6, 7), (10, 200, 37, 99, 255, 64)])
+def test_huffman_small_codes_vs_oracle(values):
+    """Codes of 2..4 bits (the super-chunk path): sizes around chunk and
+    super-chunk boundaries, symbols below 8 and above."""
+    rng = np.random.default_rng(sum(values))
+    vals = np.array(values, dtype=np.uint8)
+    for n in (2, 4095, 4097, 65535, 65537, 200_003):
+        p = rng.dirichlet(np.ones(len(vals)))
+        t = vals[rng.choice(len(vals), n, p=p)]
+        for tree in (False, True):
+            for algo, fn in (("huff_pc", O.huff_pc), ("huff_naive", O.huff_naive)):
+                try:
+                    o = fn(t, tree)
+                except RuntimeError:
+                    continue  # undefined in the reference
+                ws = pwm_amd.build(t, 0, algo, tree)
+                _same(ws, o.levels, o.bits, (values, n, tree, algo))
+                assert np.array_equal(ws.zeros(), o.zeros), (values, n, tree, algo)
+                ws.free()
