@@ -85,7 +85,9 @@ typedef struct pwmhip_ws {
   uint64_t levels;
   uint64_t text_size;
   int32_t is_tree;
-  int32_t is_huffman;
+  int32_t is_huffman;  /* 0 plain; Huffman-shaped: 1 with huff_pc's node
+                          borders where they differ from huff_naive's (no
+                          symbol queries), 2 with huff_naive's layout     */
   uint64_t* d_words;                          /* device allocation         */
   uint64_t level_offset[PWMHIP_MAX_LEVELS];   /* word offset of level l    */
   uint64_t level_bits[PWMHIP_MAX_LEVELS];     /* bits in level l           */
@@ -150,7 +152,11 @@ int pwmhip_bv_query(pwmhip_ctx* ctx, const pwmhip_rs* rs, const uint32_t* level,
  * op 1: rank(symbol a, i b)  -> occurrences of a in [0, b)
  * op 2: select(symbol a, k b)-> 0-based position of the b-th a (b >= 1),
  *                               UINT64_MAX if a occurs fewer than b times
- * Works for matrices and trees, plain and Huffman-shaped. */
+ * Works for matrices and trees, plain and Huffman-shaped (the reference has
+ * none for Huffman shapes, query_support.hpp:43,55,67): a symbol's path is
+ * its code; structures with is_huffman == 1 (huff_pc / huff_ps / huff_dd_pc
+ * node borders that differ from huff_naive's) return PWMHIP_ERR_UNSUPPORTED
+ * here and in pwmhip_rs_build. */
 int pwmhip_query(pwmhip_ctx* ctx, const pwmhip_ws* ws, const pwmhip_rs* rs,
                  const uint32_t* op, const uint64_t* a, const uint64_t* b, uint64_t n,
                  uint64_t* result, uint32_t flags);

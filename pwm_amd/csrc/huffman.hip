@@ -531,7 +531,7 @@ void build_huffman_device(pwmhip_ctx* ctx, const void* d_text, uint64_t n, uint3
                           bool pc_quirks, pwmhip_ws* ws) {
   hipStream_t st = ctx->stream;
   ws->is_tree = tree;
-  ws->is_huffman = 1;
+  ws->is_huffman = 2;  // (an empty structure has the naive layout)
   // 1. histogram, max symbol (and first occurrences for map ordering)
   const uint32_t nsym_cap = kHMaxSym;
   ctx->scratch.reset();
@@ -627,6 +627,7 @@ void build_huffman_device(pwmhip_ctx* ctx, const void* d_text, uint64_t n, uint3
   // B state of the reference's border arrays (levels < 8 / >= 8)
   std::unordered_map<uint64_t, uint64_t> bstate[2];
   bool overlap = false;
+  bool naive_layout = true;  // every node starts where huff_naive puts it
   for (uint64_t l = 1; l < L; ++l) {
     level_slot0[l] = uint32_t(slot_level.size());
     // counts per l-bit prefix of occurrences with code length > l (continuing)
@@ -673,6 +674,7 @@ void build_huffman_device(pwmhip_ctx* ctx, const void* d_text, uint64_t n, uint3
       slot_prefix.push_back(p);
       slot_count.push_back(cont[p]);
       const uint64_t s0 = pc_quirks ? bstate[l < 8 ? 0 : 1][p] : naive_start[p];
+      if (s0 != naive_start[p]) naive_layout = false;
       if (s0 + cont[p] > ct.level_sizes[l])
         throw status_error{PWMHIP_ERR_UNSUPPORTED, "reference huff_pc writes outside the level here"};
       start.push_back(s0);
@@ -704,6 +706,10 @@ void build_huffman_device(pwmhip_ctx* ctx, const void* d_text, uint64_t n, uint3
       for (uint64_t l = 0; l < ct.len[s]; ++l)
         if (!((ct.word[s] >> (ct.len[s] - 1 - l)) & 1)) zeros[l] += ct.freq[s];
   }
+
+  // 1 = Huffman-shaped, the reference's huff_pc node borders; 2 = the
+  // huff_naive layout (symbol queries can walk it)
+  ws->is_huffman = naive_layout ? 2 : 1;
 
   // 4. output allocation
   std::vector<uint64_t> bits(ct.level_sizes.begin(), ct.level_sizes.end());

@@ -97,12 +97,23 @@ struct pwmhip_ctx {
 namespace pwm {
 
 // One resident structure as the multi-structure query kernel reads it.
+// Code tables of a Huffman-shaped structure for symbol queries (device).
+struct HuffQuery {
+  const uint64_t* len;       // code length per symbol (0: absent)
+  const uint64_t* word;      // code word per symbol
+  uint64_t nsym;
+  const uint64_t* leaf_key;  // (length << 48) | word, ascending
+  const uint32_t* leaf_sym;
+  uint32_t nleaf;
+};
+
 struct QueryIndexDesc {
   const void* lv;          // per-level rank/select descriptors (device)
   const uint64_t* zeros;   // true zero counts (device)
   uint64_t n;
   uint32_t L;
   int32_t tree;
+  const HuffQuery* huff;   // device, Huffman-shaped structures only
 };
 QueryIndexDesc query_desc(const pwmhip_rs* rs);
 // Answers n symbol queries (op 0 access, 1 rank, 2 select; as pwmhip_query)

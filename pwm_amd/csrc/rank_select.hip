@@ -29,6 +29,7 @@ struct pwmhip_rs {
   void* mem = nullptr;          // one device allocation
   void* d_desc = nullptr;       // device array of level descriptors
   uint64_t true_zeros[PWMHIP_MAX_LEVELS] = {};
+  pwm::HuffQuery* d_huff = nullptr;  // Huffman-shaped: code tables (device)
 };
 
 namespace pwm {
@@ -202,6 +203,7 @@ struct QueryParams {
   int tree;
   uint64_t n;
   const uint64_t* zeros;  // true zero counts per level
+  const HuffQuery* huff;  // Huffman-shaped code tables or nullptr
   const uint32_t* op;
   const uint64_t* a;
   const uint64_t* b;
@@ -232,10 +234,27 @@ __global__ void k_bv_query(QueryParams p) {
   p.res[i] = r;
 }
 
+// Symbol of the complete code (len, word), or kNone if it is no code.
+__device__ __forceinline__ uint64_t huff_leaf(const HuffQuery* hq, uint64_t len, uint64_t word) {
+  const uint64_t key = (len << 48) | word;
+  uint32_t lo = 0, hi = hq->nleaf;  // first entry >= key
+  while (lo < hi) {
+    const uint32_t mid = (lo + hi) >> 1;
+    if (hq->leaf_key[mid] < key) lo = mid + 1;
+    else hi = mid;
+  }
+  return lo < hq->nleaf && hq->leaf_key[lo] == key ? uint64_t(hq->leaf_sym[lo]) : kNone;
+}
+
 // Symbol queries (query_support.hpp:72-178 recursions, iterative) on one
-// structure: levels lv[0..L), true zero counts, text length n.
+// structure: levels lv[0..L), true zero counts, text length n.  With code
+// tables (Huffman-shaped, huff_naive's layout) a symbol's path is its code:
+// the same recursions over its code length; the code assignment of the
+// reference (huff_codes.hpp:141-285) keeps, at every level, the codes that
+// end there behind the ones that go on, so positions below a level are
+// unchanged by the missing leaves.
 __device__ uint64_t sym_query(const LevelRS* lv, uint32_t L, int tree, uint64_t n, const uint64_t* zeros,
-                              uint32_t op, uint64_t a, uint64_t b) {
+                              const HuffQuery* hq, uint32_t op, uint64_t a, uint64_t b) {
   uint64_t res = kNone;
   if (op == 0) {  // access(a)
     if (a < n) {
@@ -245,13 +264,27 @@ __device__ uint64_t sym_query(const LevelRS* lv, uint32_t L, int tree, uint64_t 
         if (!tree) {
           const bool bit = bit_at(v, i);
           sym = (sym << 1) | bit;
+          if (hq) {
+            const uint64_t leaf = huff_leaf(hq, l + 1, sym);
+            if (leaf != kNone) {
+              sym = leaf;
+              break;
+            }
+          }
           const uint64_t r1 = rank1(v, i);
           i = bit ? zeros[l] + r1 : i - r1;
         } else {
-          const uint64_t r1s = rank1(v, s), r1e = rank1(v, e), r1i = rank1(v, s + i);
-          const uint64_t zn = (e - s) - (r1e - r1s);
           const bool bit = bit_at(v, s + i);
           sym = (sym << 1) | bit;
+          if (hq) {
+            const uint64_t leaf = huff_leaf(hq, l + 1, sym);
+            if (leaf != kNone) {
+              sym = leaf;
+              break;
+            }
+          }
+          const uint64_t r1s = rank1(v, s), r1e = rank1(v, e), r1i = rank1(v, s + i);
+          const uint64_t zn = (e - s) - (r1e - r1s);
           if (bit) { i = r1i - r1s; s = s + zn; }
           else { i = i - (r1i - r1s); e = s + zn; }
         }
@@ -259,8 +292,17 @@ __device__ uint64_t sym_query(const LevelRS* lv, uint32_t L, int tree, uint64_t 
       res = sym;
     }
   } else {
-    const uint64_t c = a;
-    if (L < 64 && (c >> L) != 0) {
+    // the symbol's path: its code (Huffman-shaped) or its L bits
+    uint64_t c = a;
+    bool absent = L < 64 && (c >> L) != 0;
+    if (hq) {
+      absent = c >= hq->nsym || hq->len[c] == 0;
+      if (!absent) {
+        L = uint32_t(hq->len[c]);
+        c = hq->word[c];
+      }
+    }
+    if (absent) {
       res = op == 1 ? 0 : (b == 0 ? 0 : kNone);
     } else if (!tree) {
       // rank: (index, offset) pair (rank_matrix); select walks back up
@@ -316,7 +358,7 @@ __device__ uint64_t sym_query(const LevelRS* lv, uint32_t L, int tree, uint64_t 
 __global__ void k_sym_query(QueryParams p) {
   const uint64_t qi = uint64_t(blockIdx.x) * blockDim.x + threadIdx.x;
   if (qi >= p.q) return;
-  p.res[qi] = sym_query(p.lv, p.L, p.tree, p.n, p.zeros, p.op[qi], p.a[qi], p.b[qi]);
+  p.res[qi] = sym_query(p.lv, p.L, p.tree, p.n, p.zeros, p.huff, p.op[qi], p.a[qi], p.b[qi]);
 }
 
 // Queries over many resident structures: query i reads structure qidx[i].
@@ -325,7 +367,7 @@ __global__ void k_multi_query(const QueryIndexDesc* desc, const uint32_t* qidx, 
   const uint64_t qi = uint64_t(blockIdx.x) * blockDim.x + threadIdx.x;
   if (qi >= q) return;
   const QueryIndexDesc d = desc[qidx[qi]];
-  res[qi] = sym_query(static_cast<const LevelRS*>(d.lv), d.L, d.tree, d.n, d.zeros, op[qi], a[qi], b[qi]);
+  res[qi] = sym_query(static_cast<const LevelRS*>(d.lv), d.L, d.tree, d.n, d.zeros, d.huff, op[qi], a[qi], b[qi]);
 }
 
 }  // namespace
@@ -344,6 +386,7 @@ QueryIndexDesc query_desc(const pwmhip_rs* rs) {
   d.n = rs->n;
   d.L = uint32_t(rs->levels);
   d.tree = rs->is_tree;
+  d.huff = rs->d_huff;
   return d;
 }
 
@@ -363,7 +406,10 @@ extern "C" {
 int pwmhip_rs_build(pwmhip_ctx* ctx, const pwmhip_ws* ws, pwmhip_rs** out) {
   if (!ctx || !ws || !out) return PWMHIP_ERR_INVALID;
   *out = nullptr;
-  if (ws->is_huffman) return PWMHIP_ERR_UNSUPPORTED;
+  // Huffman-shaped with the reference's huff_pc node borders: not a layout
+  // the symbol recursions can walk
+  if (ws->is_huffman == 1) return PWMHIP_ERR_UNSUPPORTED;
+  const bool huff = ws->is_huffman != 0;
   auto* rs = new (std::nothrow) pwmhip_rs();
   if (!rs) return PWMHIP_ERR_OOM;
   try {
@@ -385,6 +431,24 @@ int pwmhip_rs_build(pwmhip_ctx* ctx, const pwmhip_ws* ws, pwmhip_rs** out) {
       bytes += al((v.nsb + 1) * 8) + al(v.nblk * 2) + al(v.ns1 * 4) + al(v.ns0 * 4);
     }
     bytes += al(L * sizeof(LevelRS)) + al(L * 8);
+    // Huffman-shaped: code tables and the sorted complete codes
+    const uint64_t S = huff ? ws->n_codes : 0;
+    std::vector<uint64_t> hlen(S), hword(S), leaf_key;
+    std::vector<uint32_t> leaf_sym;
+    if (huff) {
+      std::vector<std::pair<uint64_t, uint32_t>> leaves;
+      for (uint64_t c = 0; c < S; ++c) {
+        hlen[c] = ws->code_len[c];
+        hword[c] = ws->code_word[c];
+        if (hlen[c]) leaves.emplace_back((hlen[c] << 48) | hword[c], uint32_t(c));
+      }
+      std::sort(leaves.begin(), leaves.end());
+      for (auto& e : leaves) {
+        leaf_key.push_back(e.first);
+        leaf_sym.push_back(e.second);
+      }
+      bytes += al(sizeof(HuffQuery)) + al(S * 8) * 2 + al(leaf_key.size() * 8 + 8) + al(leaf_sym.size() * 4 + 4);
+    }
     PWM_HIP_TRY(hipMalloc(&rs->mem, std::max<size_t>(bytes, 256)));
     char* p = static_cast<char*>(rs->mem);
     for (uint64_t l = 0; l < L; ++l) {
@@ -400,6 +464,31 @@ int pwmhip_rs_build(pwmhip_ctx* ctx, const pwmhip_ws* ws, pwmhip_rs** out) {
     rs->d_desc = p;
     p += al(L * sizeof(LevelRS));
     uint64_t* d_zeros = reinterpret_cast<uint64_t*>(p);
+    p += al(L * 8);
+    if (huff) {
+      HuffQuery hq{};
+      rs->d_huff = reinterpret_cast<HuffQuery*>(p); p += al(sizeof(HuffQuery));
+      uint64_t* d_len = reinterpret_cast<uint64_t*>(p); p += al(S * 8);
+      uint64_t* d_word = reinterpret_cast<uint64_t*>(p); p += al(S * 8);
+      uint64_t* d_key = reinterpret_cast<uint64_t*>(p); p += al(leaf_key.size() * 8 + 8);
+      uint32_t* d_sym = reinterpret_cast<uint32_t*>(p); p += al(leaf_sym.size() * 4 + 4);
+      hq.len = d_len;
+      hq.word = d_word;
+      hq.nsym = S;
+      hq.leaf_key = d_key;
+      hq.leaf_sym = d_sym;
+      hq.nleaf = uint32_t(leaf_key.size());
+      if (S) {
+        PWM_HIP_TRY(hipMemcpyAsync(d_len, hlen.data(), S * 8, hipMemcpyHostToDevice, ctx->stream));
+        PWM_HIP_TRY(hipMemcpyAsync(d_word, hword.data(), S * 8, hipMemcpyHostToDevice, ctx->stream));
+      }
+      if (!leaf_key.empty()) {
+        PWM_HIP_TRY(hipMemcpyAsync(d_key, leaf_key.data(), leaf_key.size() * 8, hipMemcpyHostToDevice, ctx->stream));
+        PWM_HIP_TRY(hipMemcpyAsync(d_sym, leaf_sym.data(), leaf_sym.size() * 4, hipMemcpyHostToDevice, ctx->stream));
+      }
+      PWM_HIP_TRY(hipMemcpyAsync(rs->d_huff, &hq, sizeof(hq), hipMemcpyHostToDevice, ctx->stream));
+      PWM_HIP_TRY(hipStreamSynchronize(ctx->stream));  // hq and the vectors are host stack objects
+    }
     // totals -> ones per level
     std::vector<uint64_t> ones(L);
     for (uint64_t l = 0; l < L; ++l)
@@ -500,7 +589,7 @@ int pwmhip_bv_query(pwmhip_ctx* ctx, const pwmhip_rs* rs, const uint32_t* level,
 int pwmhip_query(pwmhip_ctx* ctx, const pwmhip_ws* ws, const pwmhip_rs* rs, const uint32_t* op,
                  const uint64_t* a, const uint64_t* b, uint64_t n, uint64_t* result, uint32_t flags) {
   if (!ctx || !ws || !rs || (n && (!op || !a || !result))) return PWMHIP_ERR_INVALID;
-  if (ws->is_huffman) return PWMHIP_ERR_UNSUPPORTED;
+  if (ws->is_huffman == 1) return PWMHIP_ERR_UNSUPPORTED;
   const void* ins[3] = {op, a, b};
   const size_t isz[3] = {4, 8, 8};
   return run_query(ctx, n, flags, ins, isz, 3, result, [&](const void* const* d, uint64_t* r) {
@@ -511,6 +600,7 @@ int pwmhip_query(pwmhip_ctx* ctx, const pwmhip_ws* ws, const pwmhip_rs* rs, cons
     p.n = rs->n;
     p.zeros = reinterpret_cast<const uint64_t*>(static_cast<const char*>(rs->d_desc) +
                                                 ((rs->levels * sizeof(LevelRS) + 255) & ~size_t(255)));
+    p.huff = rs->d_huff;
     p.op = static_cast<const uint32_t*>(d[0]);
     p.a = static_cast<const uint64_t*>(d[1]);
     p.b = d[2] ? static_cast<const uint64_t*>(d[2]) : static_cast<const uint64_t*>(d[1]);

@@ -104,3 +104,50 @@ def test_reference_query_test_all_symbols():
         for c in range(256):
             pre[c, 1:] = np.cumsum(t == c)
         assert np.array_equal(got, pre[syms, idx].astype(np.uint64))
+
+
+def _huff_queries(rng, t, q):
+    n = t.size
+    syms = np.unique(t)
+    op = rng.integers(0, 3, q).astype(np.uint32)
+    pick = syms[rng.integers(0, syms.size, q)].astype(np.uint64)
+    absent = rng.integers(0, 8, q) == 0  # some symbols that do not occur
+    pick = np.where(absent, rng.integers(0, 1 << (8 * t.itemsize), q, dtype=np.uint64) % (int(t.max()) + 40), pick)
+    a = np.where(op == 0, rng.integers(0, n + 3, q).astype(np.uint64), pick).astype(np.uint64)
+    b = np.where(op == 1, rng.integers(0, n + 2, q), rng.integers(0, max(2, n // max(1, syms.size) * 2), q))
+    return op, a, b.astype(np.uint64)
+
+
+@pytest.mark.gpu
+@pytest.mark.parametrize("algo", ["huff_naive", "huff_pc_ss", "huff_pc"])
+def test_huffman_symbol_queries_brute_force(algo):
+    """access / rank / select on Huffman-shaped matrices and trees (the
+    reference has no query support for them) against brute force over the
+    text; huff_pc structures whose node borders differ from huff_naive's
+    report UNSUPPORTED instead."""
+    rng = np.random.default_rng(99)
+    cases = [(np.uint8, 4, 5000), (np.uint8, 30, 20000), (np.uint8, 256, 40000), (np.uint16, 900, 30000)]
+    answered = 0
+    for dt, sig, n in cases:
+        for conc in (0.3, 3.0):
+            t = rng.choice(sig, n, p=rng.dirichlet(np.ones(sig) * conc)).astype(dt)
+            for tree in (False, True):
+                try:
+                    ws = pwm_amd.build(t, 0, algo, tree)
+                except pwm_amd.PwmError as e:
+                    assert algo == "huff_pc" and e.status == pwm_amd.ERR_UNSUPPORTED
+                    continue
+                try:
+                    qs = pwm_amd.QuerySupport(ws)
+                except pwm_amd.PwmError as e:
+                    assert algo == "huff_pc" and e.status == pwm_amd.ERR_UNSUPPORTED, (algo, sig, tree)
+                    ws.free()
+                    continue
+                op, a, b = _huff_queries(rng, t, 20000)
+                got = qs.query(op, a, b)
+                want = O.symbol_queries(t, op, a, b)
+                assert np.array_equal(got, want), (algo, sig, n, conc, tree, np.flatnonzero(got != want)[:5])
+                answered += 1
+                qs.free()
+                ws.free()
+    assert answered >= (8 if algo != "huff_pc" else 1)
