@@ -38,6 +38,9 @@ CONFIGS = {
     "zipf1g_wm": ("zipf", 0xC3, 1 << 28, 4, 20, "pps", False),
     "dna4g_huffwm": ("dna", 0xC2, 1 << 32, 1, 0, "huff_pc", False),
 }
+# the same workloads with the text in pinned host memory, streamed to the
+# device in slices during the build (PCIe-bound)
+STREAM_CONFIGS = {"dna4g_stream": "dna4g_wm"}
 # random symbol queries timed after the build
 QUERIES = {"zipf1g_wm": 10**7}
 
@@ -408,7 +411,7 @@ def main():
     ap.add_argument("--steps", type=int, default=10)
     ap.add_argument("--warmup", type=int, default=3)
     ap.add_argument("--config", default="dna4g_wm",
-                    choices=sorted(CONFIGS) + sorted(BATCH_CONFIGS) + sorted(SERVE_CONFIGS))
+                    choices=sorted(CONFIGS) + sorted(STREAM_CONFIGS) + sorted(BATCH_CONFIGS) + sorted(SERVE_CONFIGS))
     ap.add_argument("--n", type=int, default=0, help="override the text length")
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--queries", type=int, default=None, help="random symbol queries to time after the build")
@@ -438,25 +441,41 @@ def main():
     import torch
     import pwm_amd
 
-    kind, seed, n, width, levels, algo, tree = CONFIGS[args.config]
+    streamed = args.config in STREAM_CONFIGS
+    kind, seed, n, width, levels, algo, tree = CONFIGS[STREAM_CONFIGS.get(args.config, args.config)]
     if args.n:
         n = args.n
     t0 = time.perf_counter()
     text = pwm_amd.generate(kind, seed, n, width, levels)
     gen_s = time.perf_counter() - t0
     dev = torch.device("cuda:0")
-    d_text = torch.from_numpy(text).to(dev)
+    h2d_gbps = None
+    if streamed:
+        # pinned host text; the build copies it in slices
+        pinned = torch.empty(n * width, dtype=torch.uint8, pin_memory=True)
+        pinned.numpy()[:] = text.view(np.uint8)
+        src = pinned.numpy().view(text.dtype)
+        d_tmp = torch.empty(n * width, dtype=torch.uint8, device=dev)
+        d_tmp.copy_(pinned, non_blocking=True)
+        torch.cuda.synchronize()
+        t0 = time.perf_counter()
+        d_tmp.copy_(pinned, non_blocking=True)
+        torch.cuda.synchronize()
+        h2d_gbps = round(n * width / (time.perf_counter() - t0) / 1e9, 2)
+        del d_tmp
+    else:
+        src = torch.from_numpy(text).to(dev)
     torch.cuda.synchronize()
     ctx = pwm_amd.Context(0)
 
     ws = None
     for _ in range(args.warmup):
-        ws = pwm_amd.build(d_text, levels, algo, tree, ctx=ctx, reuse=ws)
+        ws = pwm_amd.build(src, levels, algo, tree, ctx=ctx, reuse=ws, stream=streamed)
     torch.cuda.synchronize()
     dev_ms = []
     t0 = time.perf_counter()
     for _ in range(args.steps):
-        ws = pwm_amd.build(d_text, levels, algo, tree, ctx=ctx, reuse=ws)
+        ws = pwm_amd.build(src, levels, algo, tree, ctx=ctx, reuse=ws, stream=streamed)
         dev_ms.append(ctx.last_timing())
     torch.cuda.synchronize()
     wall = time.perf_counter() - t0
@@ -495,6 +514,8 @@ def main():
             "device_ms_per_build": round(dev_total, 4),
             "device_ms_phases": {"local_build": phases[0], "offsets": phases[1], "stitch": phases[2]},
             "symbols_per_s": round(n / (ms_per_step / 1e3), 1),
+            "text": "pinned host memory, streamed in 256 Mi-symbol slices" if streamed else "resident in HBM",
+            "plain_h2d_copy_GBps": h2d_gbps,
             "build_roofline": whole,
             "generate_s": round(gen_s, 2)},
         "roofline": roof,

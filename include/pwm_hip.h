@@ -77,6 +77,11 @@ enum pwmhip_algo {
                                        /* sequence when the geometry repeats   */
 #define PWMHIP_FLAG_AUTOTUNE 32u       /* batches: time the launch variants of  */
                                        /* a new length/level bucket, keep best */
+#define PWMHIP_FLAG_STREAM 64u         /* host text copied in slices while the  */
+                                       /* local build runs; only the output and */
+                                       /* scratch live in HBM (2..4 levels;     */
+                                       /* the analogue of wx_ppc_ie,            */
+                                       /* external_memory/wx_ppc_ie.hpp:21-65)  */
 
 typedef struct pwmhip_ctx pwmhip_ctx;
 

@@ -40,6 +40,7 @@ FLAG_CHECK_RANGE = 4
 FLAG_REUSE_OUTPUT = 8
 FLAG_GRAPH = 16
 FLAG_AUTOTUNE = 32
+FLAG_STREAM = 64
 
 # algorithm ids (pwm_hip.h enum pwmhip_algo)
 ALGOS = {
@@ -281,10 +282,13 @@ class WaveletStructure:
             pass
 
 
-def build(text, levels, algo="ppc", is_tree=False, ctx=None, check_range=False, size=None, reuse=None):
+def build(text, levels, algo="ppc", is_tree=False, ctx=None, check_range=False, size=None, reuse=None,
+          stream=False):
     """pwmhip_build: construct one structure; `text` is a numpy array (host)
     or a device tensor (torch on cuda:0).  `reuse` is an earlier
-    WaveletStructure whose device memory is taken over."""
+    WaveletStructure whose device memory is taken over.  `stream` (host
+    texts, 2..4 levels): the text is copied in slices while the local build
+    runs and is never resident in HBM as a whole."""
     ctx = ctx or default_context()
     arg = _text_arg(text)
     ptr, n, w, flags = arg[0], arg[1], arg[2], arg[3]
@@ -295,6 +299,8 @@ def build(text, levels, algo="ppc", is_tree=False, ctx=None, check_range=False, 
         flags |= FLAG_TREE
     if check_range:
         flags |= FLAG_CHECK_RANGE
+    if stream:
+        flags |= FLAG_STREAM
     ws = _WS()
     if reuse is not None and reuse._ws is not None:
         ws = reuse._ws

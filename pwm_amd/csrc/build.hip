@@ -2691,8 +2691,63 @@ void build_passes_device(pwmhip_ctx* ctx, const void* d_text, uint64_t n, uint32
   for (uint32_t l = 0; l < L; ++l) ws->zeros[l] = tree ? 0 : zeros[l];
 }
 
+namespace {
+
+// Symbols per streamed slice: a multiple of the super-chunk
+// (PWMHIP_STREAM_SLICE_SYMS overrides the 256 Mi default).
+uint64_t stream_slice_syms() {
+  const char* e = std::getenv("PWMHIP_STREAM_SLICE_SYMS");
+  uint64_t v = e ? std::strtoull(e, nullptr, 10) : (uint64_t(256) << 20);
+  v = std::max<uint64_t>(v, kSuperSyms);
+  return v / kSuperSyms * kSuperSyms;
+}
+
+// Local build of a host text in slices: slice k is copied into staging
+// buffer k % 2 on the copy stream while the build of slice k - 1 runs on
+// the context's stream; a buffer is refilled once the build that read it
+// has finished.  Every slice but the last is whole super-chunks, so its
+// outputs (level 0, chunk-local rows, counts) sit at fixed offsets.
+void stream_local_build(pwmhip_ctx* ctx, const void* h_text, uint64_t n, uint32_t width, uint32_t L,
+                        const K1Params& k1) {
+  if (!ctx->copy_stream) PWM_HIP_TRY(hipStreamCreateWithFlags(&ctx->copy_stream, hipStreamNonBlocking));
+  for (auto& e : ctx->sev)
+    if (!e) PWM_HIP_TRY(hipEventCreateWithFlags(&e, hipEventDisableTiming));
+  const uint64_t slice = stream_slice_syms();
+  const uint64_t bytes = std::min<uint64_t>(slice, n) * width;
+  ctx->text_scratch.ensure(2 * ((bytes + 255) & ~uint64_t(255)) + 64);
+  char* buf[2] = {static_cast<char*>(ctx->text_scratch.base),
+                  static_cast<char*>(ctx->text_scratch.base) + ((bytes + 255) & ~uint64_t(255))};
+  const char* src = static_cast<const char*>(h_text);
+  uint64_t k = 0;
+  for (uint64_t off = 0; off < n; off += slice, ++k) {
+    const uint32_t b = uint32_t(k & 1);
+    const uint64_t m = std::min<uint64_t>(slice, n - off);
+    if (k >= 2) PWM_HIP_TRY(hipStreamWaitEvent(ctx->copy_stream, ctx->sev[2 + b], 0));
+    PWM_HIP_TRY(hipMemcpyAsync(buf[b], src + off * width, m * width, hipMemcpyHostToDevice, ctx->copy_stream));
+    PWM_HIP_TRY(hipEventRecord(ctx->sev[b], ctx->copy_stream));
+    PWM_HIP_TRY(hipStreamWaitEvent(ctx->stream, ctx->sev[b], 0));
+    K1Params ks = k1;
+    ks.text = buf[b];
+    ks.n = m;
+    ks.nchunks = (m + kChunkSyms - 1) / kChunkSyms;
+    ks.out_level0 = k1.out_level0 + off / 64;
+    ks.temp_levels = k1.temp_levels + (off / kSuperSyms) * kSuperWords;
+    ks.counts = k1.counts + (off / kSuperSyms) * k1.nslot;
+    switch (L) {
+      case 2: launch_super<2>(ctx, ks); break;
+      case 3: launch_super<3>(ctx, ks); break;
+      default: launch_super<4>(ctx, ks); break;
+    }
+    PWM_HIP_TRY(hipEventRecord(ctx->sev[2 + b], ctx->stream));
+  }
+}
+
+}  // namespace
+
 void build_plain_device(pwmhip_ctx* ctx, const void* d_text, uint64_t n, uint32_t width,
-                        uint32_t L, bool tree, bool check_range, pwmhip_ws* ws) {
+                        uint32_t L, bool tree, bool check_range, pwmhip_ws* ws, const void* h_stream) {
+  if (h_stream && (L < 2 || L > 4 || small_k1_disabled() || super_disabled()))
+    throw status_error{PWMHIP_ERR_UNSUPPORTED, "streamed texts: 2 to 4 levels"};
   if (L > 4 && !small_k1_disabled()) {
     build_passes_device(ctx, d_text, n, width, L, tree, check_range, ws);
     return;
@@ -2802,7 +2857,9 @@ void build_plain_device(pwmhip_ctx* ctx, const void* d_text, uint64_t n, uint32_
     k1.sym_or = sym_or;
     k1.nchunks = nchunks;
     k1.lut = compress_table(ctx);
-    if (sup) {
+    if (h_stream) {
+      stream_local_build(ctx, h_stream, n, width, L, k1);
+    } else if (sup) {
       switch (L) {
         case 2: launch_super<2>(ctx, k1); break;
         case 3: launch_super<3>(ctx, k1); break;

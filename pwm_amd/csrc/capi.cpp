@@ -120,6 +120,7 @@ int pwmhip_destroy(pwmhip_ctx* ctx) {
   if (!ctx) return PWMHIP_ERR_INVALID;
   (void)hipSetDevice(ctx->device);
   if (ctx->stream) (void)hipStreamSynchronize(ctx->stream);
+  if (ctx->copy_stream) (void)hipStreamSynchronize(ctx->copy_stream);
   ctx->scratch.release();
   ctx->text_scratch.release();
   ctx->rs_scratch.release();
@@ -130,6 +131,9 @@ int pwmhip_destroy(pwmhip_ctx* ctx) {
   if (ctx->graph) (void)hipGraphDestroy(ctx->graph);
   for (auto& e : ctx->ev)
     if (e) (void)hipEventDestroy(e);
+  for (auto& e : ctx->sev)
+    if (e) (void)hipEventDestroy(e);
+  if (ctx->copy_stream) (void)hipStreamDestroy(ctx->copy_stream);
   if (ctx->stream) (void)hipStreamDestroy(ctx->stream);
   delete ctx;
   return PWMHIP_OK;
@@ -155,10 +159,12 @@ int pwmhip_build(pwmhip_ctx* ctx, const void* text, uint64_t size, uint32_t word
   out->is_huffman = huff;
   if (!huff && (levels == 0 || levels > 8 * word_width || levels > PWMHIP_MAX_LEVELS)) return PWMHIP_ERR_INVALID;
   if (size > 0 && !text) return PWMHIP_ERR_INVALID;
+  const bool stream = (flags & PWMHIP_FLAG_STREAM) && !(flags & PWMHIP_FLAG_TEXT_ON_DEVICE) && size > 0;
+  if (stream && huff) return PWMHIP_ERR_UNSUPPORTED;
   return pwm::run_guarded("build", [&] {
     PWM_HIP_TRY(hipSetDevice(ctx->device));
     const void* d_text = text;
-    if (size > 0 && !(flags & PWMHIP_FLAG_TEXT_ON_DEVICE)) {
+    if (size > 0 && !(flags & PWMHIP_FLAG_TEXT_ON_DEVICE) && !stream) {
       ctx->text_scratch.ensure(size * word_width + 64);
       PWM_HIP_TRY(hipMemcpyAsync(ctx->text_scratch.base, text, size * word_width, hipMemcpyHostToDevice,
                                  ctx->stream));
@@ -188,8 +194,8 @@ int pwmhip_build(pwmhip_ctx* ctx, const void* text, uint64_t size, uint32_t word
     pwm::alloc_structure(out, levels, bits.data(), reuse_ptr, reuse_bytes);
     reuse_ptr = nullptr;
     try {
-      pwm::build_plain_device(ctx, d_text, size, word_width, levels, tree,
-                              (flags & PWMHIP_FLAG_CHECK_RANGE) != 0, out);
+      pwm::build_plain_device(ctx, stream ? nullptr : d_text, size, word_width, levels, tree,
+                              (flags & PWMHIP_FLAG_CHECK_RANGE) != 0, out, stream ? text : nullptr);
     } catch (...) {
       (void)hipFree(out->d_words);
       out->d_words = nullptr;

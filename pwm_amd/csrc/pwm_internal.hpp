@@ -92,11 +92,13 @@ struct pwmhip_ctx {
   hipGraph_t graph = nullptr;
   hipGraphExec_t gexec = nullptr;
   uint64_t gkey[12] = {};
+  // streamed host texts: copy stream, slice events (copied, consumed)
+  hipStream_t copy_stream = nullptr;
+  hipEvent_t sev[4] = {nullptr, nullptr, nullptr, nullptr};
 };
 
 namespace pwm {
 
-// One resident structure as the multi-structure query kernel reads it.
 // Code tables of a Huffman-shaped structure for symbol queries (device).
 struct HuffQuery {
   const uint64_t* len;       // code length per symbol (0: absent)
@@ -107,6 +109,7 @@ struct HuffQuery {
   uint32_t nleaf;
 };
 
+// One resident structure as the multi-structure query kernel reads it.
 struct QueryIndexDesc {
   const void* lv;          // per-level rank/select descriptors (device)
   const uint64_t* zeros;   // true zero counts (device)
@@ -129,8 +132,11 @@ const uint8_t* compress_table(pwmhip_ctx* ctx);
 // Builds `levels` levels of a wavelet matrix (tree=false) or tree (tree=true)
 // for a device-resident text into ws (which the caller has sized).  Fills
 // ws->zeros with the true zero counts (matrix).  Throws on error.
+// h_stream: a host text copied to the device in slices while the local
+// build runs (d_text unused); only 2..4 levels (ERR_UNSUPPORTED otherwise).
 void build_plain_device(pwmhip_ctx* ctx, const void* d_text, uint64_t n, uint32_t width,
-                        uint32_t levels, bool tree, bool check_range, pwmhip_ws* ws);
+                        uint32_t levels, bool tree, bool check_range, pwmhip_ws* ws,
+                        const void* h_stream = nullptr);
 
 // Builds T texts (device pointers) with one launch per phase.  All
 // structures live in one allocation owned by outs[0] (alloc_bytes > 0, the
