@@ -207,7 +207,8 @@ int pwmhip_build_batch(pwmhip_ctx* ctx, const void* const* texts, const uint64_t
  * index (levels + rank/select directories) resident until released, and
  * answers waiting queries of all ready indices together in one launch.
  * Each result equals what the same request gives alone (pwmhip_build +
- * pwmhip_query).  Plain algorithms only (Huffman shapes have no queries).
+ * pwmhip_query).  Plain algorithms only (a Huffman shape's size is known only
+ * after its code plan, which the admission estimate would need).
  * Host buffers passed in (texts, query arrays, result arrays) must stay
  * valid until the request completes. */
 typedef struct pwmhip_server pwmhip_server;

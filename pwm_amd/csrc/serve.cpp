@@ -481,7 +481,7 @@ int pwmhip_server_build(pwmhip_server* s, const void* text, uint64_t size, uint3
                         uint32_t algo, uint32_t flags, uint64_t* index_id) {
   if (!s || !index_id || (size && !text)) return PWMHIP_ERR_INVALID;
   if (word_width != 1 && word_width != 2 && word_width != 4 && word_width != 8) return PWMHIP_ERR_INVALID;
-  if (algo > PWMHIP_ALGO_NAIVE) return PWMHIP_ERR_UNSUPPORTED;  // Huffman shapes: no queries
+  if (algo > PWMHIP_ALGO_NAIVE) return PWMHIP_ERR_UNSUPPORTED;  // plain shapes (sizes known at admission)
   if (levels == 0 || levels > 8 * word_width || levels > PWMHIP_MAX_LEVELS) return PWMHIP_ERR_INVALID;
   if (flags & (PWMHIP_FLAG_TEXT_ON_DEVICE | PWMHIP_FLAG_REUSE_OUTPUT)) return PWMHIP_ERR_INVALID;
   Index x;
