@@ -928,21 +928,22 @@ constexpr int kSuper = 16;                                  // chunks per super-
 constexpr uint64_t kSuperSyms = uint64_t(kSuper) * kChunkSyms;
 constexpr uint32_t kSuperWords = kSuper * kChunkWords;      // words per sub-level
 
-template <int G, bool WIDE>
-__global__ __launch_bounds__(64 * kSuper) void k_local_super(K1Params p) {
+template <int G, bool WIDE, int WAVES>
+__global__ __launch_bounds__(64 * WAVES) void k_local_super(K1Params p) {
   static_assert(G >= 2 && G <= 4, "two to four levels");
-  constexpr int WAVES = kSuper;
+  constexpr uint64_t kUnitSyms = uint64_t(WAVES) * kChunkSyms;
+  constexpr uint32_t kUnitWords = WAVES * kChunkWords;
   constexpr int NSUB = G - 1;      // sub-levels 1..G-1
   constexpr int JC = G - 1;        // last sub-level
   constexpr int NQ = 1 << JC;      // its buckets
   constexpr int NF = NQ + 1;       // bucket counts + the last level's zeros
   constexpr int NS = (NF + 1) / 2; // packed scans
   __shared__ __align__(16) uint8_t lut[65536];
-  __shared__ uint64_t img[NSUB][kSuperWords];
+  __shared__ uint64_t img[NSUB][kUnitWords];
   __shared__ uint32_t wt[WAVES][NQ];
   __shared__ unsigned long long red[G + 1];  // block sums: zeros per level, symbol OR
   if (threadIdx.x <= G) red[threadIdx.x] = 0;
-  for (uint32_t i = threadIdx.x; i < NSUB * kSuperWords; i += 64 * WAVES) (&img[0][0])[i] = 0;
+  for (uint32_t i = threadIdx.x; i < NSUB * kUnitWords; i += 64 * WAVES) (&img[0][0])[i] = 0;
 
   const int lane = threadIdx.x & 63;
   const int wid = threadIdx.x >> 6;
@@ -1078,12 +1079,12 @@ __global__ __launch_bounds__(64 * kSuper) void k_local_super(K1Params p) {
     }
     lds_sync();  // the image is complete
 
-    const uint64_t s0 = sc * kSuperSyms;
-    const uint32_t nws = uint32_t((min<uint64_t>(kSuperSyms, p.n - s0) + 63) >> 6);
+    const uint64_t s0 = sc * kUnitSyms;
+    const uint32_t nws = uint32_t((min<uint64_t>(kUnitSyms, p.n - s0) + 63) >> 6);
 #pragma unroll
     for (int j = 0; j < NSUB; ++j) {
       if (threadIdx.x < nws)
-        st_stream(p.temp_levels + j * p.temp_stride + sc * kSuperWords + threadIdx.x, brev64(img[j][threadIdx.x]));
+        st_stream(p.temp_levels + j * p.temp_stride + sc * kUnitWords + threadIdx.x, brev64(img[j][threadIdx.x]));
       img[j][threadIdx.x] = 0;  // cleared before the next super-chunk's first barrier
     }
     if (uwid == 0) {
@@ -1107,6 +1108,231 @@ __global__ __launch_bounds__(64 * kSuper) void k_local_super(K1Params p) {
       if (red[j]) atomicAdd(p.zeros_out + j, red[j]);
     if (red[G]) atomicOr(p.sym_or, red[G]);
   }
+}
+
+// ---- K1 over super-chunk units of many texts ---------------------------------
+// Batches of long texts: text t owns units [base[t], base[t+1]) of up to 16
+// chunks (the single-text super-chunk build per unit).  Per-unit metadata
+// (first symbol, symbols left in the text, level-0 word, text) comes from a
+// table, so that a unit needs one round of independent loads, issued a unit
+// ahead.  A workgroup takes units from the queue one at a time, the next
+// one a unit ahead so that its text is prefetched; zero counts are reduced
+// per unit from the waves' bucket totals (one atomic per level and unit).
+struct UnitMeta {
+  uint64_t text;  // address of the unit's first symbol
+  uint64_t rem;   // symbols of the text from the unit's first on
+  uint64_t lvl0;  // word of the unit's first level-0 bit in the output block
+  uint64_t t;     // text
+};
+
+// One thread per text: the metadata of its units, and unit -> text.
+__global__ void k_unit_meta(const uint64_t* base, const uint64_t* text, const uint64_t* n, const uint64_t* out0,
+                            uint32_t ntexts, uint32_t width, uint64_t unit_syms, UnitMeta* meta,
+                            uint32_t* unit_text) {
+  const uint32_t t = blockIdx.x * blockDim.x + threadIdx.x;
+  if (t >= ntexts) return;
+  for (uint64_t u = base[t]; u < base[t + 1]; ++u) {
+    const uint64_t lu = u - base[t];
+    UnitMeta m;
+    m.text = text[t] + lu * unit_syms * width;
+    m.rem = n[t] - lu * unit_syms;
+    m.lvl0 = out0[t] + lu * (unit_syms / 64);
+    m.t = t;
+    meta[u] = m;
+    unit_text[u] = t;
+  }
+}
+
+template <int G, bool WIDE, int WAVES>
+__global__ __launch_bounds__(64 * WAVES) void k_local_super_batch(K1Params p, const UnitMeta* meta) {
+  static_assert(G >= 2 && G <= 4, "two to four levels");
+  constexpr uint64_t kUnitSyms = uint64_t(WAVES) * kChunkSyms;
+  constexpr uint32_t kUnitWords = WAVES * kChunkWords;
+  constexpr int NSUB = G - 1;
+  constexpr int JC = G - 1;
+  constexpr int NQ = 1 << JC;
+  constexpr int NF = NQ + 1;       // bucket counts + the last level's zeros
+  constexpr int NS = (NF + 1) / 2;
+  __shared__ __align__(16) uint8_t lut[65536];
+  __shared__ uint64_t img[NSUB][kUnitWords];
+  __shared__ uint32_t wt[WAVES][NF];
+  __shared__ unsigned long long red_or;
+  __shared__ uint64_t s_unit[3];
+  if (threadIdx.x == 0) {
+    red_or = 0;
+    s_unit[0] = atomicAdd(p.work, 1u);
+    s_unit[1] = atomicAdd(p.work, 1u);
+  }
+  for (uint32_t i = threadIdx.x; i < NSUB * kUnitWords; i += 64 * WAVES) (&img[0][0])[i] = 0;
+  const int lane = threadIdx.x & 63;
+  const int wid = threadIdx.x >> 6;
+  const int uwid = __builtin_amdgcn_readfirstlane(wid);
+  load_lut<WAVES>(lut, p.lut);  // (ends with a barrier)
+
+  uint64_t my_or = 0;
+  const uint64_t U = p.nchunks;  // units
+  auto get = [&](uint64_t u) {
+    UnitMeta m{0, 0, 0, 0};
+    if (u < U) m = gmem(meta)[u];
+    return m;
+  };
+  // this wave's chunk of a unit: symbols and whether it is a whole aligned chunk
+  auto chunk_syms = [&](const UnitMeta& m) -> uint32_t {
+    const uint64_t c0 = uint64_t(wid) * kChunkSyms;
+    return m.rem > c0 ? uint32_t(min<uint64_t>(kChunkSyms, m.rem - c0)) : 0u;
+  };
+  auto full = [&](const UnitMeta& m) {
+    return !WIDE && (m.text & 15) == 0 && m.rem >= uint64_t(wid + 1) * kChunkSyms;
+  };
+  auto load_chunk = [&](const UnitMeta& m, uint4* d) {
+    const auto src = gmem(reinterpret_cast<const uint4*>(m.text + uint64_t(wid) * kChunkSyms + 64ull * lane));
+#pragma unroll
+    for (int i = 0; i < 4; ++i) d[i] = src[i];
+  };
+  uint4 pf[4];
+  uint64_t u = s_unit[0], un = s_unit[1];
+  UnitMeta cur = get(u), nxt = get(un);
+  if (u < U && full(cur)) load_chunk(cur, pf);
+
+  while (u < U) {
+    if (threadIdx.x == 0) s_unit[2] = atomicAdd(p.work, 1u);  // read after the first barrier
+    const uint32_t Ec = chunk_syms(cur);
+    const uint32_t nw = (Ec + 63) >> 6;
+    const int cnt = max(0, min(64, int(Ec) - 64 * lane));
+
+    uint64_t P0[G];
+#pragma unroll
+    for (int k = 0; k < G; ++k) P0[k] = 0;
+    const bool pf_next = un < U && full(nxt);
+    if (full(cur)) {
+      planes_from_bytes<G>(p, pf, P0, my_or);
+      if (pf_next) load_chunk(nxt, pf);  // prefetch
+    } else {
+      if (cnt > 0) {
+        const uint64_t e0 = uint64_t(wid) * kChunkSyms + 64ull * lane;
+        const void* tx = reinterpret_cast<const void*>(cur.text);
+        if constexpr (!WIDE) {
+          load_planes_text<G, uint8_t>(p, tx, e0, cnt, P0, my_or);
+        } else {
+          switch (p.width) {
+            case 2: load_planes_text<G, uint16_t>(p, tx, e0, cnt, P0, my_or); break;
+            case 4: load_planes_text<G, uint32_t>(p, tx, e0, cnt, P0, my_or); break;
+            default: load_planes_text<G, uint64_t>(p, tx, e0, cnt, P0, my_or); break;
+          }
+        }
+      }
+      if (pf_next) load_chunk(nxt, pf);
+    }
+    const uint64_t vm = cnt >= 64 ? ~0ull : (cnt <= 0 ? 0ull : ((1ull << cnt) - 1));
+    if (lane < int(nw)) st_stream(p.out_level0 + cur.lvl0 + uint64_t(wid) * kChunkWords + lane, brev64(P0[0]));
+
+    uint64_t M[NQ];
+    M[0] = vm;
+#pragma unroll
+    for (int i = 0; i < JC; ++i) {
+#pragma unroll
+      for (int q = (1 << i) - 1; q >= 0; --q) {
+        M[q + (1 << i)] = M[q] & P0[i];
+        M[q] = M[q] & ~P0[i];
+      }
+    }
+    uint32_t f[2 * NS];
+#pragma unroll
+    for (int q = 0; q < NQ; ++q) f[q] = __popcll(M[q]);
+    f[NQ] = __popcll(~P0[G - 1] & vm);
+    if (NF < 2 * NS) f[2 * NS - 1] = 0;
+    uint32_t ex[2 * NS], tot[2 * NS];
+#pragma unroll
+    for (int s2 = 0; s2 < NS; ++s2) {
+      const uint32_t v = f[2 * s2] | (f[2 * s2 + 1] << 16);
+      const uint32_t inc = wave_incl_scan(v);
+      const uint32_t tt = __builtin_amdgcn_readlane(inc, 63);
+      const uint32_t e = inc - v;
+      ex[2 * s2] = e & 0xffffu;
+      ex[2 * s2 + 1] = e >> 16;
+      tot[2 * s2] = tt & 0xffffu;
+      tot[2 * s2 + 1] = tt >> 16;
+    }
+    if (lane < NF) {
+      uint32_t v = 0;
+#pragma unroll
+      for (int q = 0; q < NF; ++q) v = lane == q ? tot[q] : v;
+      wt[wid][lane] = v;
+    }
+    lds_sync();  // every wave's totals are there; the image was cleared
+
+    // the unit after the next: its metadata loads overlap the rest of this unit
+    const uint64_t unn = s_unit[2];
+    const UnitMeta nn = get(unn);
+
+    uint32_t stot[NF], spre[NQ];
+#pragma unroll
+    for (int q = 0; q < NF; ++q) {
+      const uint32_t v = lane < WAVES ? wt[lane][q] : 0u;
+      const uint32_t inc = wave_incl_scan(v);
+      stot[q] = __builtin_amdgcn_readlane(inc, WAVES - 1);
+      if (q < NQ) spre[q] = __builtin_amdgcn_readlane(inc - v, uwid);
+    }
+    if (threadIdx.x == 0) {
+      // the unit's zeros per level: level j < G-1 has bit j == 0 in the
+      // last sub-level's buckets q with bit j clear
+      unsigned long long* zd = p.b_zeros + cur.t * kMaxGroup;
+#pragma unroll
+      for (int j = 0; j < JC; ++j) {
+        uint32_t z = 0;
+#pragma unroll
+        for (int q = 0; q < NQ; ++q)
+          if (!((q >> j) & 1)) z += stot[q];
+        if (z) atomicAdd(zd + j, (unsigned long long)z);
+      }
+      if (stot[NQ]) atomicAdd(zd + (G - 1), (unsigned long long)stot[NQ]);
+    }
+    uint32_t cslot[2 * NQ - 2];
+#pragma unroll
+    for (int j = 1; j <= JC; ++j) {
+      const int nb = 1 << j;
+      uint32_t start = 0;
+#pragma unroll
+      for (int q = 0; q < nb; ++q) {
+        uint32_t T = 0, E = 0;
+#pragma unroll
+        for (int r = q; r < NQ; r += nb) {
+          T += stot[r];
+          E += spre[r] + ex[r];
+        }
+        uint64_t mq = vm;
+#pragma unroll
+        for (int i = 0; i < j; ++i) mq &= ((q >> i) & 1) ? P0[i] : ~P0[i];
+        or_bits(img[j - 1], start + E, compress64h(P0[j], mq, lut), __popcll(mq));
+        cslot[nb - 2 + q] = T;
+        start += T;
+      }
+    }
+    lds_sync();  // the image is complete
+
+    const uint32_t nws = uint32_t((min<uint64_t>(kUnitSyms, cur.rem) + 63) >> 6);
+#pragma unroll
+    for (int j = 0; j < NSUB; ++j) {
+      if (threadIdx.x < nws)
+        st_stream(p.temp_levels + j * p.temp_stride + u * kUnitWords + threadIdx.x, brev64(img[j][threadIdx.x]));
+      img[j][threadIdx.x] = 0;  // cleared before the next unit's first barrier
+    }
+    if (uwid == 0) {
+      uint32_t v = 0;
+#pragma unroll
+      for (int i = 0; i < 2 * NQ - 2; ++i) v = lane == i ? cslot[i] : v;
+      if (lane < 2 * NQ - 2) gmem(p.counts)[u * p.nslot + lane] = v;
+    }
+    u = un;
+    cur = nxt;
+    un = unn;
+    nxt = nn;
+  }
+#pragma unroll
+  for (int d = 32; d >= 1; d >>= 1) my_or |= __shfl_xor(my_or, d, 64);
+  if (lane == 0 && my_or) atomicOr(&red_or, (unsigned long long)my_or);
+  __syncthreads();
+  if (threadIdx.x == 0 && red_or) atomicOr(p.sym_or, red_or);
 }
 
 // ---- Huffman-shaped local build over super-chunks -------------------------
@@ -1989,6 +2215,40 @@ __global__ __launch_bounds__(256) void k_gather_batch(BatchGatherParams p) {
                p.nwin, stage[threadIdx.x >> 6]);
 }
 
+// Batch stitch over super-chunk runs: waves belong to texts by wave_base
+// (steps of 128 words, nwin steps per wave); a wave finds its first run in
+// its text's table slice with a 64-way ballot search.
+__global__ __launch_bounds__(256) void k_gather_batch_wide(BatchGatherParams p) {
+  __shared__ unsigned long long stage[4][64];
+  const uint32_t lane = threadIdx.x & 63;
+  const uint64_t wv = uint64_t(blockIdx.x) * 4 + (threadIdx.x >> 6);
+  if (wv >= p.wave_base[p.ntexts]) return;  // whole waves; no barrier below
+  uint32_t lo_t = 0, hi_t = p.ntexts;  // wave_base[lo] <= wv < wave_base[hi]
+  while (hi_t - lo_t > 1) {
+    const uint32_t mid = (lo_t + hi_t) >> 1;
+    if (p.wave_base[mid] <= wv) lo_t = mid;
+    else hi_t = mid;
+  }
+  const uint32_t t = lo_t;
+  const uint64_t W0 = (wv - p.wave_base[t]) * 128 * p.nwin;
+  const uint64_t toff = uint64_t((1u << p.j) - 2) * p.nchunks + p.chunk_base[t] * (1ull << p.j);
+  const uint64_t* td = p.tab_dst + toff;
+  const uint64_t NE = p.level_ne[uint64_t(t) * 8 + p.j];
+  const uint64_t n = p.text_n[t];
+  const uint64_t X = W0 << 6;
+  if (X >= n || NE == 0) return;
+  uint64_t lo = 0, hi = NE;  // last entry with dst <= X
+  while (hi - lo > 1) {
+    const uint64_t step = (hi - lo + 63) >> 6;
+    const uint64_t cand = lo + uint64_t(lane + 1) * step;
+    const bool pred = cand < hi && td[cand] <= X;
+    lo += uint64_t(__popcll(__ballot(pred))) * step;
+    hi = min(hi, lo + step);
+  }
+  gather_range_wide(td, p.tab_src + toff, NE, p.src + (p.j - 1) * p.src_stride, p.out + p.out0[t] + p.j * p.stride[t],
+                    n, W0, p.nwin, lo, stage[threadIdx.x >> 6]);
+}
+
 // Fused offsets for the run table: per (tile of 1024 chunks, slot) the sums of
 // counts and of non-empty indicators ...
 __global__ __launch_bounds__(256) void k_tile_sums2(const uint32_t* counts, uint64_t* tsum, uint64_t* tnz,
@@ -2249,7 +2509,7 @@ __global__ __launch_bounds__(1024) void k_batch_apply_runs(const uint32_t* count
                                                            const uint64_t* tb_z, const uint64_t* gstart,
                                                            const uint64_t* nebase, uint64_t* tab_dst,
                                                            uint64_t* tab_src, uint64_t nchunks, uint32_t nslot,
-                                                           uint64_t ntiles) {
+                                                           uint64_t ntiles, uint64_t csyms) {
   __shared__ uint64_t wsum[32];
   uint64_t pref, nzpref;
   uint32_t x;
@@ -2264,7 +2524,7 @@ __global__ __launch_bounds__(1024) void k_batch_apply_runs(const uint32_t* count
   for (uint32_t b = s0; b < slot; ++b) loc += counts[c * nslot + b];
   const uint64_t e = uint64_t(s0) * nchunks + chunk_base[t] * (1ull << j) + nebase[ts] + (nzpref - tb_z[ts]);
   tab_dst[e] = gstart[ts] + (pref - tb_v[ts]);
-  tab_src[e] = c * kChunkSyms + loc;
+  tab_src[e] = c * csyms + loc;
 }
 
 // Bucket starts of every (text, sub-level); grid (ntexts, jmax), 64 threads.
@@ -2362,15 +2622,31 @@ uint32_t wide_gather_steps() {
   return v;
 }
 
+// Chunks per super-chunk unit of the plain builds: 16 (one workgroup of 16
+// waves per CU) or 8 (two workgroups of 8 per CU); PWMHIP_SUPER_WAVES.
+uint32_t super_waves() {
+  static const uint32_t v = [] {
+    const char* e = std::getenv("PWMHIP_SUPER_WAVES");
+    return e && std::atoi(e) == 8 ? 8u : 16u;
+  }();
+  return v;
+}
+
+template <int G, int WAVES>
+void launch_super_w(pwmhip_ctx* ctx, const K1Params& prm) {
+  const uint64_t nsuper = (prm.nchunks + WAVES - 1) / WAVES;
+  const uint64_t grid = std::min<uint64_t>(nsuper, uint64_t(ctx->num_cus) * (16 / WAVES));
+  if (prm.width == 1)
+    hipLaunchKernelGGL((k_local_super<G, false, WAVES>), dim3(uint32_t(grid)), dim3(64 * WAVES), 0, ctx->stream, prm);
+  else
+    hipLaunchKernelGGL((k_local_super<G, true, WAVES>), dim3(uint32_t(grid)), dim3(64 * WAVES), 0, ctx->stream, prm);
+  PWM_HIP_TRY(hipGetLastError());
+}
+
 template <int G>
 void launch_super(pwmhip_ctx* ctx, const K1Params& prm) {
-  const uint64_t nsuper = (prm.nchunks + kSuper - 1) / kSuper;
-  const uint64_t grid = std::min<uint64_t>(nsuper, uint64_t(ctx->num_cus));
-  if (prm.width == 1)
-    hipLaunchKernelGGL((k_local_super<G, false>), dim3(uint32_t(grid)), dim3(64 * kSuper), 0, ctx->stream, prm);
-  else
-    hipLaunchKernelGGL((k_local_super<G, true>), dim3(uint32_t(grid)), dim3(64 * kSuper), 0, ctx->stream, prm);
-  PWM_HIP_TRY(hipGetLastError());
+  if (super_waves() == 8) launch_super_w<G, 8>(ctx, prm);
+  else launch_super_w<G, 16>(ctx, prm);
 }
 
 }  // namespace
@@ -2390,6 +2666,29 @@ void launch_huff_super(pwmhip_ctx* ctx, const HuffSuperParams& hp) {
 }
 
 namespace {
+
+template <int W>
+void launch_super_batch_w(pwmhip_ctx* ctx, const K1Params& prm, uint32_t L, const UnitMeta* meta) {
+  const uint32_t grid =
+      uint32_t(std::min<uint64_t>(std::max<uint64_t>(prm.nchunks, 1), uint64_t(ctx->num_cus) * (16 / W)));
+  const bool wide = prm.width != 1;
+  const dim3 gr(grid), bl(64 * W);
+  hipStream_t s = ctx->stream;
+  switch (L * 2 + (wide ? 1 : 0)) {
+    case 4: hipLaunchKernelGGL((k_local_super_batch<2, false, W>), gr, bl, 0, s, prm, meta); break;
+    case 5: hipLaunchKernelGGL((k_local_super_batch<2, true, W>), gr, bl, 0, s, prm, meta); break;
+    case 6: hipLaunchKernelGGL((k_local_super_batch<3, false, W>), gr, bl, 0, s, prm, meta); break;
+    case 7: hipLaunchKernelGGL((k_local_super_batch<3, true, W>), gr, bl, 0, s, prm, meta); break;
+    case 8: hipLaunchKernelGGL((k_local_super_batch<4, false, W>), gr, bl, 0, s, prm, meta); break;
+    default: hipLaunchKernelGGL((k_local_super_batch<4, true, W>), gr, bl, 0, s, prm, meta); break;
+  }
+  PWM_HIP_TRY(hipGetLastError());
+}
+
+void launch_super_batch(pwmhip_ctx* ctx, const K1Params& prm, uint32_t L, const UnitMeta* meta) {
+  if (super_waves() == 8) launch_super_batch_w<8>(ctx, prm, L, meta);
+  else launch_super_batch_w<16>(ctx, prm, L, meta);
+}
 
 template <int NP>
 void launch_k1(pwmhip_ctx* ctx, const K1Params& prm) {
@@ -2698,8 +2997,9 @@ namespace {
 uint64_t stream_slice_syms() {
   const char* e = std::getenv("PWMHIP_STREAM_SLICE_SYMS");
   uint64_t v = e ? std::strtoull(e, nullptr, 10) : (uint64_t(256) << 20);
-  v = std::max<uint64_t>(v, kSuperSyms);
-  return v / kSuperSyms * kSuperSyms;
+  const uint64_t us = uint64_t(super_waves()) * kChunkSyms;
+  v = std::max<uint64_t>(v, us);
+  return v / us * us;
 }
 
 // Local build of a host text in slices: slice k is copied into staging
@@ -2731,8 +3031,9 @@ void stream_local_build(pwmhip_ctx* ctx, const void* h_text, uint64_t n, uint32_
     ks.n = m;
     ks.nchunks = (m + kChunkSyms - 1) / kChunkSyms;
     ks.out_level0 = k1.out_level0 + off / 64;
-    ks.temp_levels = k1.temp_levels + (off / kSuperSyms) * kSuperWords;
-    ks.counts = k1.counts + (off / kSuperSyms) * k1.nslot;
+    const uint64_t us = uint64_t(super_waves()) * kChunkSyms;
+    ks.temp_levels = k1.temp_levels + (off / us) * (us / 64);
+    ks.counts = k1.counts + (off / us) * k1.nslot;
     switch (L) {
       case 2: launch_super<2>(ctx, ks); break;
       case 3: launch_super<3>(ctx, ks); break;
@@ -2759,9 +3060,10 @@ void build_plain_device(pwmhip_ctx* ctx, const void* d_text, uint64_t n, uint32_
   const bool tree_direct = tree && !multi;
   // 2..4 levels from a text: runs per super-chunk; otherwise per chunk
   const bool sup = L >= 2 && L <= 4 && !small_k1_disabled() && !super_disabled();
-  const uint64_t units = sup ? (nchunks + kSuper - 1) / kSuper : nchunks;
-  const uint64_t unit_syms = sup ? kSuperSyms : kChunkSyms;
-  const uint64_t tstride = sup ? units * kSuperWords + 64 : stride;
+  const uint32_t sw = super_waves();
+  const uint64_t units = sup ? (nchunks + sw - 1) / sw : nchunks;
+  const uint64_t unit_syms = sup ? uint64_t(sw) * kChunkSyms : kChunkSyms;
+  const uint64_t tstride = sup ? units * (unit_syms / 64) + 64 : stride;
   // wide stitch: 128-word steps, wide_nstep per wave, one grid for all sub-levels
   const bool wide = sup && !wide_gather_disabled();
   const uint32_t wide_nstep = wide_gather_steps();
@@ -3005,6 +3307,7 @@ struct BatchTune {
 };
 
 const BatchTune kTuneCandidates[] = {{4, 8}, {16, 8}, {64, 8}, {4, 32}, {16, 32}, {64, 32}};
+const BatchTune kTuneCandidatesUnits[] = {{1, 4}, {1, 8}, {1, 16}};
 
 }  // namespace
 
@@ -3017,9 +3320,16 @@ void build_batch_device(pwmhip_ctx* ctx, const void* const* d_texts, const uint6
   const uint32_t nslot = jmax ? (2u << jmax) - 2u : 0;
   std::vector<uint64_t> cb(T + 1), out0(T), stride(T), wave_base(T + 1);
   std::vector<uint64_t> tptr(T), tn(T);
+  // texts of 32 Ki symbols and more on average: units of 16 chunks (the
+  // super-chunk build and stitch); otherwise one unit per chunk
+  uint64_t total_syms = 0;
+  for (uint64_t t = 0; t < T; ++t) total_syms += sizes[t];
+  const bool sunits = L >= 2 && L <= 4 && T > 0 && total_syms / T >= 32768 && !small_k1_disabled() &&
+                      !super_disabled();
+  const uint64_t unit_syms = sunits ? uint64_t(super_waves()) * kChunkSyms : kChunkSyms;
   uint64_t words = 0;
   for (uint64_t t = 0; t < T; ++t) {
-    cb[t + 1] = cb[t] + (sizes[t] + kChunkSyms - 1) / kChunkSyms;
+    cb[t + 1] = cb[t] + (sizes[t] + unit_syms - 1) / unit_syms;
     stride[t] = level_stride(sizes[t]);
     out0[t] = words;
     words += sizes[t] ? stride[t] * L : 0;
@@ -3069,7 +3379,7 @@ void build_batch_device(pwmhip_ctx* ctx, const void* const* d_texts, const uint6
 
   // gather waves per text for a number of windows per wave
   auto waves_for_nwin = [&](uint32_t nwin) {
-    const uint64_t per_wave = 64ull * nwin;
+    const uint64_t per_wave = (sunits ? 128ull : 64ull) * nwin;
     uint64_t wv = 0;
     for (uint64_t t = 0; t < T; ++t) {
       wave_base[t] = wv;
@@ -3079,7 +3389,7 @@ void build_batch_device(pwmhip_ctx* ctx, const void* const* d_texts, const uint6
     return wv;
   };
 
-  const uint64_t tstride = C * kChunkWords + 64;
+  const uint64_t tstride = C * (unit_syms / 64) + 64;
   const uint64_t ntiles = (C + kTile - 1) / kTile;
   const size_t sz_cnt = std::max<size_t>(1, C * nslot);
   arena sizing;
@@ -3112,6 +3422,7 @@ void build_batch_device(pwmhip_ctx* ctx, const void* const* d_texts, const uint6
     v.push_back(A.take<uint32_t>(std::max<size_t>(1, C))); // 23 chunk -> text
     for (int k = 0; k < 4; ++k)                            // 24..27 text bounds
       v.push_back(A.take<uint64_t>(std::max<size_t>(1, T * nslot)));
+    v.push_back(A.take<UnitMeta>(sunits ? std::max<size_t>(1, C) : 1));  // 28 unit metadata
   };
   std::vector<void*> b;
   carve(sizing, b);
@@ -3186,7 +3497,14 @@ void build_batch_device(pwmhip_ctx* ctx, const void* const* d_texts, const uint6
     k1.work = static_cast<unsigned int*>(b[22]);
     k1.grab = tu.grab;
     k1.b_chunk_text = static_cast<uint32_t*>(b[23]);
-    dispatch_k1(ctx, k1);
+    if (sunits) {
+      hipLaunchKernelGGL(k_unit_meta, dim3(uint32_t((T + 255) / 256)), dim3(256), 0, s, U64(0), U64(1), U64(2),
+                         U64(3), uint32_t(T), width, unit_syms, static_cast<UnitMeta*>(b[28]),
+                         static_cast<uint32_t*>(b[23]));
+      launch_super_batch(ctx, k1, L, static_cast<const UnitMeta*>(b[28]));
+    } else {
+      dispatch_k1(ctx, k1);
+    }
     if (events) PWM_HIP_TRY(hipEventRecord(ctx->ev[1], s));
     if (nslot) {
       const uint32_t* counts = static_cast<const uint32_t*>(b[8]);
@@ -3206,7 +3524,7 @@ void build_batch_device(pwmhip_ctx* ctx, const void* const* d_texts, const uint6
                          U64(16), U64(17), U64(18), nslot, tree ? 1 : 0);
       hipLaunchKernelGGL(k_batch_apply_runs, dim3(uint32_t(ntiles), nslot), dim3(1024), 0, s, counts, U64(9),
                          U64(10), ctext, U64(0), U64(24), U64(25), U64(16), U64(17), U64(19), U64(20), C, nslot,
-                         ntiles);
+                         ntiles, unit_syms);
       if (events) PWM_HIP_TRY(hipEventRecord(ctx->ev[2], s));
       for (uint32_t j = 1; j <= jmax; ++j) {
         BatchGatherParams bp{};
@@ -3225,7 +3543,8 @@ void build_batch_device(pwmhip_ctx* ctx, const void* const* d_texts, const uint6
         bp.j = j;
         bp.nchunks = C;
         bp.nwin = tu.nwin;
-        hipLaunchKernelGGL(k_gather_batch, dim3(uint32_t((wv + 3) / 4)), dim3(256), 0, s, bp);
+        if (sunits) hipLaunchKernelGGL(k_gather_batch_wide, dim3(uint32_t((wv + 3) / 4)), dim3(256), 0, s, bp);
+        else hipLaunchKernelGGL(k_gather_batch, dim3(uint32_t((wv + 3) / 4)), dim3(256), 0, s, bp);
       }
     } else if (events) {
       PWM_HIP_TRY(hipEventRecord(ctx->ev[2], s));
@@ -3237,15 +3556,20 @@ void build_batch_device(pwmhip_ctx* ctx, const void* const* d_texts, const uint6
   // tunables of this length / level bucket (mean text length, rounded to a
   // power of two, and the number of levels)
   uint32_t lg = 0;
-  for (uint64_t m = std::max<uint64_t>(1, words_for(cb[T] * kChunkSyms) / T); m > 1; m >>= 1) ++lg;
-  const uint64_t bucket = (uint64_t(L) << 8) | lg;
-  BatchTune tu{16, kGatherWindows};
+  for (uint64_t m = std::max<uint64_t>(1, words_for(total_syms) / T); m > 1; m >>= 1) ++lg;
+  const uint64_t bucket = (uint64_t(L) << 8) | lg | (sunits ? 1ull << 16 : 0ull);
+  // super-chunk units: one unit per grab, `nwin` = 128-word steps per wave
+  BatchTune tu = sunits ? BatchTune{1, 8} : BatchTune{16, kGatherWindows};
   auto it = ctx->tune.find(bucket);
   if (it != ctx->tune.end()) {
     tu = BatchTune{it->second.first, it->second.second};
   } else if (flags & PWMHIP_FLAG_AUTOTUNE) {
     float best = 0;
-    for (const BatchTune& cand : kTuneCandidates) {
+    const BatchTune* cands = sunits ? kTuneCandidatesUnits : kTuneCandidates;
+    const size_t ncand = sunits ? sizeof(kTuneCandidatesUnits) / sizeof(BatchTune)
+                                : sizeof(kTuneCandidates) / sizeof(BatchTune);
+    for (size_t ci = 0; ci < ncand; ++ci) {
+      const BatchTune& cand = cands[ci];
       const uint64_t wv = upload(cand.nwin);
       enqueue(cand, wv, true);
       PWM_HIP_TRY(hipStreamSynchronize(s));

@@ -143,3 +143,43 @@ def test_batch_graph_replay_and_autotune():
         del got
     finally:
         ctx.close()
+
+
+@pytest.mark.gpu
+@pytest.mark.parametrize("w,L", [(1, 2), (1, 3), (1, 4), (2, 3), (4, 4)])
+def test_batch_long_texts_super_units(w, L):
+    """Batches of long texts (mean above 32 Ki symbols: units of 16 chunks,
+    the super-chunk build and stitch) equal single builds; empty and tiny
+    texts mixed in; graph replay and autotuning on the same path."""
+    rng = np.random.default_rng(500 + 10 * w + L)
+    sizes = np.clip(rng.lognormal(np.log(60000), 1.2, 48), 1, 900_000).astype(np.int64)
+    sizes[[3, 17]] = 0
+    sizes[[5, 30]] = [1, 65537]
+    sizes[40] = 65536 * 3
+    texts = _texts(rng, sizes, w, L)
+    assert sizes.sum() / sizes.size >= 32768
+    for tree in (False, True):
+        got = pwm_amd.build_batch(texts, L, "ppc", tree, check_range=True)
+        for i, t in enumerate(texts):
+            ref = pwm_amd.build(t, L, "ppc", tree)
+            _same(got[i], ref, (w, L, tree, i, t.size))
+            ref.free()
+        for i in range(0, len(texts), 7):
+            if 0 < texts[i].size < 20000:
+                o = O.naive(texts[i], L, tree)
+                for l, (x, y) in enumerate(zip(got[i].bvs(), o.levels)):
+                    assert np.array_equal(x, y), (w, L, tree, i, l)
+        del got
+    ctx = pwm_amd.Context(0)
+    try:
+        prev = None
+        for rep in range(3):
+            got = pwm_amd.build_batch(texts, L, "pps", False, ctx=ctx, graph=True, autotune=True, reuse=prev)
+            for i in range(0, len(texts), 5):
+                ref = pwm_amd.build(texts[i], L, "pps", False, ctx=ctx)
+                _same(got[i], ref, ("graph", rep, i))
+                ref.free()
+            prev = got
+        del got, prev
+    finally:
+        ctx.close()
