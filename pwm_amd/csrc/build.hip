@@ -1570,13 +1570,15 @@ __global__ __launch_bounds__(256) void k_scatter(ScatterParams p) {
         mine &= ((q >> b) & 1) ? B : ~B;
       }
       const uint32_t rank = mbcnt64(mine);
+      // one wave's LDS accesses execute in program order, and the compiler
+      // keeps these (possibly aliasing) ones in order: every lane reads its
+      // bucket's position before the bucket's last lane advances it, and
+      // the next step reads the advanced positions
       const uint64_t base = run[q];
-      wave_sync();  // every lane has read its bucket's position
       if (valid) {
         out[base + rank] = TO(uint64_t(x[u]) & low);
         if (rank + 1 == uint32_t(__popcll(mine))) run[q] = base + rank + 1;  // the bucket's last lane
       }
-      wave_sync();
     }
   }
 }
