@@ -56,13 +56,28 @@ __global__ __launch_bounds__(256) void k_leaf_hist(const T* text, uint64_t n, ui
   const uint32_t K = local ? (1u << L) : 0u;
   for (uint32_t i = threadIdx.x; i < K; i += blockDim.x) lh[i] = 0;
   __syncthreads();
-  for (uint64_t i = uint64_t(blockIdx.x) * blockDim.x + threadIdx.x; i < n; i += uint64_t(gridDim.x) * blockDim.x) {
-    const uint64_t c = text[i];
+  auto add = [&](uint64_t c) {
     if (L >= 64 || (c >> L) == 0) {
       if (local) atomicAdd(&lh[c], 1u);
       else atomicAdd(&hist[c], 1ull);
     }
+  };
+  // 16-byte loads (16 / sizeof(T) symbols each) when the text is aligned
+  constexpr uint32_t PER = 16 / sizeof(T);
+  const uint64_t gid = uint64_t(blockIdx.x) * blockDim.x + threadIdx.x;
+  const uint64_t step = uint64_t(gridDim.x) * blockDim.x;
+  const uint64_t nv = (reinterpret_cast<uintptr_t>(text) & 15) == 0 ? n / PER : 0;
+  const uint4* tv = reinterpret_cast<const uint4*>(text);
+  for (uint64_t i = gid; i < nv; i += step) {
+    union {
+      uint4 v;
+      T x[PER];
+    } u;
+    u.v = tv[i];
+#pragma unroll
+    for (uint32_t k = 0; k < PER; ++k) add(u.x[k]);
   }
+  for (uint64_t i = nv * PER + gid; i < n; i += step) add(text[i]);
   __syncthreads();
   for (uint32_t i = threadIdx.x; i < K; i += blockDim.x)
     if (lh[i]) atomicAdd(&hist[i], (unsigned long long)lh[i]);
@@ -99,13 +114,19 @@ __global__ void k_gather_runs(uint64_t* dst, const uint64_t* src, const uint64_t
 void wm_to_wt_device(pwmhip_ctx* ctx, const void* d_text, uint64_t n, uint32_t width, uint32_t L,
                      pwmhip_ws* ws) {
   if (L > 24) throw status_error{PWMHIP_ERR_UNSUPPORTED, "tree above 24 levels"};
+  if (L < 2 || n == 0) return;
   hipStream_t st = ctx->stream;
   const uint64_t K = 1ull << L;
+  const uint64_t stride = level_stride(n);
+  // scratch: leaf histogram, every level's run table, the reordered levels
+  const uint64_t tab = 2 * (K + 2 * uint64_t(L));
   ctx->scratch.reset();
-  ctx->scratch.ensure(K * 8 + 4 * (K + 2) * 8 + 4096);
+  ctx->scratch.ensure((K + tab + uint64_t(L - 1) * stride) * 8 + 4096);
   unsigned long long* d_hist = ctx->scratch.take<unsigned long long>(K);
+  uint64_t* d_tab = ctx->scratch.take<uint64_t>(tab);
+  uint64_t* d_lv = ctx->scratch.take<uint64_t>(uint64_t(L - 1) * stride);
   PWM_HIP_TRY(hipMemsetAsync(d_hist, 0, K * 8, st));
-  const uint32_t grid = uint32_t(std::min<uint64_t>((n + 255) / 256, uint64_t(ctx->num_cus) * 16));
+  const uint32_t grid = uint32_t(std::min<uint64_t>((n + 4095) / 4096, uint64_t(ctx->num_cus) * 8));
   switch (width) {
     case 1: hipLaunchKernelGGL(k_leaf_hist<uint8_t>, dim3(grid), dim3(256), 0, st, (const uint8_t*)d_text, n, L, d_hist); break;
     case 2: hipLaunchKernelGGL(k_leaf_hist<uint16_t>, dim3(grid), dim3(256), 0, st, (const uint16_t*)d_text, n, L, d_hist); break;
@@ -113,23 +134,19 @@ void wm_to_wt_device(pwmhip_ctx* ctx, const void* d_text, uint64_t n, uint32_t w
     default: hipLaunchKernelGGL(k_leaf_hist<uint64_t>, dim3(grid), dim3(256), 0, st, (const uint64_t*)d_text, n, L, d_hist); break;
   }
   PWM_HIP_TRY(hipGetLastError());
-  std::vector<uint64_t> cnt(K);
-  PWM_HIP_TRY(hipMemcpyAsync(cnt.data(), d_hist, K * 8, hipMemcpyDeviceToHost, st));
+  std::vector<uint64_t> pc(K);
+  PWM_HIP_TRY(hipMemcpyAsync(pc.data(), d_hist, K * 8, hipMemcpyDeviceToHost, st));
   PWM_HIP_TRY(hipStreamSynchronize(st));
-  uint64_t* d_dstart = ctx->scratch.take<uint64_t>(K / 2 + 2);
-  uint64_t* d_sstart = ctx->scratch.take<uint64_t>(K / 2 + 2);
 
-  // new allocation in tree order; level 0 is the text order in both shapes
-  pwmhip_ws tw = *ws;
-  std::vector<uint64_t> bits(L, n);
-  alloc_structure(&tw, L, bits.data());
-  PWM_HIP_TRY(hipMemcpyAsync(tw.d_words + tw.level_offset[0], ws->d_words + ws->level_offset[0],
-                             words_for(n) * 8, hipMemcpyDeviceToDevice, st));
-  std::vector<uint64_t> pc(cnt), ds, ss;
+  // run tables of every level l >= 1, in destination (tree) order: runs of
+  // the non-empty l-bit prefixes p, from matrix position (prefixes in
+  // bit-reversed order) to tree position (numeric order)
+  std::vector<uint64_t> all;  // per level: ds[0..nruns], ss[0..nruns]
+  std::vector<uint64_t> off(L), nr(L);
+  all.reserve(tab);
   for (uint32_t l = L - 1; l >= 1; --l) {
-    // pc: counts of every l-bit prefix
     std::vector<uint64_t> up(1ull << l);
-    for (uint64_t p = 0; p < up.size(); ++p) up[p] = pc[2 * p] + pc[2 * p + 1];
+    for (uint64_t q = 0; q < up.size(); ++q) up[q] = pc[2 * q] + pc[2 * q + 1];
     pc.swap(up);
     const uint64_t nb = 1ull << l;
     std::vector<uint64_t> wm_start(nb);
@@ -140,8 +157,7 @@ void wm_to_wt_device(pwmhip_ctx* ctx, const void* d_text, uint64_t n, uint32_t w
       wm_start[p] = acc;
       acc += pc[p];
     }
-    ds.clear();
-    ss.clear();
+    std::vector<uint64_t> ds, ss;
     acc = 0;
     for (uint64_t p = 0; p < nb; ++p) {
       if (pc[p]) {
@@ -152,20 +168,33 @@ void wm_to_wt_device(pwmhip_ctx* ctx, const void* d_text, uint64_t n, uint32_t w
     }
     ds.push_back(n);
     ss.push_back(0);
-    const uint64_t nruns = ds.size() - 1;
-    PWM_HIP_TRY(hipMemcpyAsync(d_dstart, ds.data(), ds.size() * 8, hipMemcpyHostToDevice, st));
-    PWM_HIP_TRY(hipMemcpyAsync(d_sstart, ss.data(), ss.size() * 8, hipMemcpyHostToDevice, st));
-    const uint64_t nw = words_for(n);
-    hipLaunchKernelGGL(k_gather_runs, dim3(uint32_t((nw + 255) / 256)), dim3(256), 0, st,
-                       tw.d_words + tw.level_offset[l], ws->d_words + ws->level_offset[l], d_dstart, d_sstart,
-                       nruns, n);
-    PWM_HIP_TRY(hipGetLastError());
-    PWM_HIP_TRY(hipStreamSynchronize(st));  // host vectors are reused next level
+    off[l] = all.size();
+    nr[l] = ds.size() - 1;
+    all.insert(all.end(), ds.begin(), ds.end());
+    all.insert(all.end(), ss.begin(), ss.end());
   }
-  (void)hipFree(ws->d_words);
-  ws->d_words = tw.d_words;
-  ws->alloc_bytes = tw.alloc_bytes;
-  for (uint32_t l = 0; l < L; ++l) ws->level_offset[l] = tw.level_offset[l];
+  PWM_HIP_TRY(hipMemcpyAsync(d_tab, all.data(), all.size() * 8, hipMemcpyHostToDevice, st));
+  const uint64_t nw = words_for(n);
+  for (uint32_t l = 1; l < L; ++l) {
+    const uint64_t* ds = d_tab + off[l];
+    hipLaunchKernelGGL(k_gather_runs, dim3(uint32_t((nw + 255) / 256)), dim3(256), 0, st,
+                       d_lv + uint64_t(l - 1) * stride, ws->d_words + ws->level_offset[l], ds, ds + nr[l] + 1,
+                       nr[l], n);
+  }
+  PWM_HIP_TRY(hipGetLastError());
+  // the tree levels replace the matrix levels in place (one copy when
+  // levels 1.. are contiguous at one stride)
+  bool contiguous = true;
+  for (uint32_t l = 2; l < L; ++l) contiguous &= ws->level_offset[l] == ws->level_offset[1] + (l - 1) * stride;
+  if (contiguous) {
+    PWM_HIP_TRY(hipMemcpyAsync(ws->d_words + ws->level_offset[1], d_lv, uint64_t(L - 1) * stride * 8,
+                               hipMemcpyDeviceToDevice, st));
+  } else {
+    for (uint32_t l = 1; l < L; ++l)
+      PWM_HIP_TRY(hipMemcpyAsync(ws->d_words + ws->level_offset[l], d_lv + uint64_t(l - 1) * stride, nw * 8,
+                                 hipMemcpyDeviceToDevice, st));
+  }
+  PWM_HIP_TRY(hipStreamSynchronize(st));  // (the host run table)
 }
 
 void copy_bits_device(hipStream_t st, uint64_t* dst, const uint64_t* src, uint64_t dst_off,

@@ -41,6 +41,7 @@
 #include <type_traits>
 #include <cstring>
 #include <algorithm>
+#include <chrono>
 
 namespace pwm {
 
@@ -2940,6 +2941,18 @@ uint32_t width_for_bits(uint32_t bits) { return bits <= 8 ? 1 : bits <= 16 ? 2 :
 
 }  // namespace
 
+// Trees above one pass: the matrix levels' buckets moved into numeric order.
+// Its time (host-side, it ends with a synchronisation) counts as stitch
+// time of the build.
+static void tree_order_timed(pwmhip_ctx* ctx, const void* d_text, uint64_t n, uint32_t width, uint32_t L,
+                             pwmhip_ws* ws) {
+  const auto t0 = std::chrono::steady_clock::now();
+  wm_to_wt_device(ctx, d_text, n, width, L, ws);
+  const double dt = std::chrono::duration<double, std::milli>(std::chrono::steady_clock::now() - t0).count();
+  ctx->last_ms[2] += dt;
+  ctx->last_ms[3] += dt;
+}
+
 // Passes of at most 4 levels over the text; between passes the symbols are
 // reordered into the next pass's level order, so every pass sees few
 // buckets per chunk (long runs) whatever the number of levels.  Trees are
@@ -3094,7 +3107,7 @@ void build_passes_device(pwmhip_ctx* ctx, const void* d_text, uint64_t n, uint32
     throw status_error{PWMHIP_ERR_SYMBOL_RANGE, "symbol >= 2^levels"};
   uint64_t zeros[64];
   std::memcpy(zeros, ctx->pinned, L * 8);
-  if (tree) wm_to_wt_device(ctx, d_text, n, width, L, ws);
+  if (tree) tree_order_timed(ctx, d_text, n, width, L, ws);
   ws->n_zeros = tree ? 0 : L;
   for (uint32_t l = 0; l < L; ++l) ws->zeros[l] = tree ? 0 : zeros[l];
 }
@@ -3400,7 +3413,7 @@ void build_plain_device(pwmhip_ctx* ctx, const void* d_text, uint64_t n, uint32_
   if (check_range && L < 64 && (ctx->pinned[64] >> L) != 0)
     throw status_error{PWMHIP_ERR_SYMBOL_RANGE, "symbol >= 2^levels"};
   // trees above one pass: matrix levels with buckets moved into numeric order
-  if (tree && multi) wm_to_wt_device(ctx, d_text, n, width, L, ws);
+  if (tree && multi) tree_order_timed(ctx, d_text, n, width, L, ws);
   ws->n_zeros = tree ? 0 : L;
   for (uint32_t l = 0; l < L; ++l) ws->zeros[l] = tree ? 0 : ctx->pinned[l];
 }
