@@ -231,6 +231,9 @@ struct K1Params {
   unsigned int* work;        // dynamic chunk queue (atomic counter), or nullptr
   uint32_t grab;             // chunks taken from the queue at a time
   uint32_t* b_chunk_text;    // batch mode: text of every chunk (written here)
+  uint32_t* chunk_sc;        // super-chunk passes followed by a reorder: every
+                             // chunk's counts of the 2^G buckets of the order
+                             // after the pass, [c * 2^G + q]
   const uint8_t* lut;        // 64 KiB compress table in device memory
 };
 
@@ -929,7 +932,10 @@ constexpr int kSuper = 16;                                  // chunks per super-
 constexpr uint64_t kSuperSyms = uint64_t(kSuper) * kChunkSyms;
 constexpr uint32_t kSuperWords = kSuper * kChunkWords;      // words per sub-level
 
-template <int G, bool WIDE, int WAVES>
+// With SC (a pass followed by a symbol reorder) every wave also counts its
+// chunk's 2^G buckets of the order after the pass (chunk_sc), and the unit's
+// sums of them follow the sub-level counts in its count row.
+template <int G, bool WIDE, int WAVES, bool SC = false>
 __global__ __launch_bounds__(64 * WAVES) void k_local_super(K1Params p) {
   static_assert(G >= 2 && G <= 4, "two to four levels");
   constexpr uint64_t kUnitSyms = uint64_t(WAVES) * kChunkSyms;
@@ -942,6 +948,7 @@ __global__ __launch_bounds__(64 * WAVES) void k_local_super(K1Params p) {
   __shared__ __align__(16) uint8_t lut[65536];
   __shared__ uint64_t img[NSUB][kUnitWords];
   __shared__ uint32_t wt[WAVES][NQ];
+  __shared__ uint32_t wsc[SC ? WAVES : 1][SC ? 2 * NQ : 1];  // per wave: next-order bucket counts
   __shared__ unsigned long long red[G + 1];  // block sums: zeros per level, symbol OR
   if (threadIdx.x <= G) red[threadIdx.x] = 0;
   for (uint32_t i = threadIdx.x; i < NSUB * kUnitWords; i += 64 * WAVES) (&img[0][0])[i] = 0;
@@ -1046,7 +1053,34 @@ __global__ __launch_bounds__(64 * WAVES) void k_local_super(K1Params p) {
       for (int q = 0; q < NQ; ++q) v = lane == q ? tot[q] : v;
       wt[wid][lane] = v;
     }
+    if constexpr (SC) {
+      // bucket q of the last sub-level splits by the last level's bit into
+      // next-order buckets q and q + NQ (counted in pairs, 16 bits each)
+      uint32_t lo = 0, hi = 0;
+#pragma unroll
+      for (int q = 0; q < NQ; ++q) {
+        const uint32_t v = __popcll(M[q] & ~P0[G - 1]) | (uint32_t(__popcll(M[q] & P0[G - 1])) << 16);
+        const uint32_t t = __builtin_amdgcn_readlane(wave_incl_scan(v), 63);
+        lo = lane == q ? (t & 0xffffu) : lo;
+        hi = lane == q ? (t >> 16) : hi;
+      }
+      if (lane < NQ) {
+        wsc[wid][lane] = lo;
+        wsc[wid][lane + NQ] = hi;
+        if (c < p.nchunks) {
+          gmem(p.chunk_sc)[c * (2 * NQ) + lane] = lo;
+          gmem(p.chunk_sc)[c * (2 * NQ) + lane + NQ] = hi;
+        }
+      }
+    }
     lds_sync();  // every wave's totals are there; the image was cleared
+    uint32_t unit_sc = 0;  // wave 0, lane q < 2^G: the unit's next-order bucket q
+    if constexpr (SC) {
+      if (uwid == 0 && lane < 2 * NQ) {
+#pragma unroll
+        for (int w = 0; w < WAVES; ++w) unit_sc += wsc[w][lane];
+      }
+    }
 
     // the super-chunk's totals and this wave's prefix, per last-level bucket
     uint32_t stot[NQ], spre[NQ];
@@ -1093,6 +1127,9 @@ __global__ __launch_bounds__(64 * WAVES) void k_local_super(K1Params p) {
 #pragma unroll
       for (int i = 0; i < 2 * NQ - 2; ++i) v = lane == i ? cslot[i] : v;
       if (lane < 2 * NQ - 2) gmem(p.counts)[sc * p.nslot + lane] = v;
+      if constexpr (SC) {
+        if (lane < 2 * NQ) gmem(p.counts)[sc * p.nslot + 2 * NQ - 2 + lane] = unit_sc;
+      }
     }
   }
   if (lane == 0) {
@@ -2649,6 +2686,21 @@ __global__ void k_fill_u32(unsigned int* p, uint32_t n, uint32_t v) {
   if (i < n) p[i] = v;
 }
 
+// Positions of every chunk's buckets in the order after a super-chunk pass:
+// the unit's bucket position plus the counts of the unit's earlier chunks.
+// One block per unit, thread (k, q) = chunk k of the unit, bucket q.
+__global__ __launch_bounds__(256) void k_chunk_dstpos(const uint64_t* unit_pos, const uint32_t* chunk_sc,
+                                                      uint64_t* chunk_pos, uint64_t nchunks, uint32_t nq,
+                                                      uint32_t cpu) {
+  const uint64_t u = blockIdx.x;
+  const uint32_t k = threadIdx.x / nq, q = threadIdx.x % nq;
+  const uint64_t c = u * cpu + k;
+  if (k >= cpu || c >= nchunks) return;
+  uint64_t pos = unit_pos[u * nq + q];
+  for (uint32_t k2 = 0; k2 < k; ++k2) pos += chunk_sc[(u * cpu + k2) * nq + q];
+  chunk_pos[c * nq + q] = pos;
+}
+
 }  // namespace
 
 void launch_scan_slots(hipStream_t s, const uint32_t* counts, uint64_t* pref, uint64_t* totals,
@@ -2743,6 +2795,32 @@ void launch_super_w(pwmhip_ctx* ctx, const K1Params& prm) {
   else
     hipLaunchKernelGGL((k_local_super<G, true, WAVES>), dim3(uint32_t(grid)), dim3(64 * WAVES), 0, ctx->stream, prm);
   PWM_HIP_TRY(hipGetLastError());
+}
+
+// A pass over W-chunk units followed by a reorder (next-order counts).
+template <int G, int W>
+void launch_super_sc(pwmhip_ctx* ctx, const K1Params& prm) {
+  const uint64_t nsuper = (prm.nchunks + W - 1) / W;
+  const uint64_t grid = std::min<uint64_t>(nsuper, uint64_t(ctx->num_cus) * (16 / W));
+  if (prm.width == 1)
+    hipLaunchKernelGGL((k_local_super<G, false, W, true>), dim3(uint32_t(grid)), dim3(64 * W), 0, ctx->stream, prm);
+  else
+    hipLaunchKernelGGL((k_local_super<G, true, W, true>), dim3(uint32_t(grid)), dim3(64 * W), 0, ctx->stream, prm);
+  PWM_HIP_TRY(hipGetLastError());
+}
+
+// One pass of g (2..4) levels over W-chunk units, with (sc) or without the
+// next-order counts.
+template <int W>
+void launch_pass_units(pwmhip_ctx* ctx, const K1Params& prm, uint32_t g, bool sc) {
+  switch (g * 2 + (sc ? 1 : 0)) {
+    case 4: launch_super_w<2, W>(ctx, prm); break;
+    case 5: launch_super_sc<2, W>(ctx, prm); break;
+    case 6: launch_super_w<3, W>(ctx, prm); break;
+    case 7: launch_super_sc<3, W>(ctx, prm); break;
+    case 8: launch_super_w<4, W>(ctx, prm); break;
+    default: launch_super_sc<4, W>(ctx, prm); break;
+  }
 }
 
 template <int G>
@@ -2970,9 +3048,26 @@ void build_passes_device(pwmhip_ctx* ctx, const void* d_text, uint64_t n, uint32
   const uint64_t ntiles_max = (nchunks + kTile - 1) / kTile;
   const size_t sz_cnt = std::max<size_t>(1, nchunks * nslot_max);
   const size_t text_bytes = (n * width + 255) & ~size_t(255);
+  // runs per 16-chunk unit (the single-text super-chunk K1, with the
+  // next-order counts per chunk) and the wide stitch
+  // (with 8 or more 16-chunk units per CU; below that the per-unit barriers
+  // cost more than the longer runs save: 64 Mi bytes, 8 levels, 0.383 ms in
+  // chunks against 0.398 ms in units).  PWMHIP_PASS_UNITS=1 / 0, read at
+  // every call, takes units / chunks whatever the size.
+  const char* pu = std::getenv("PWMHIP_PASS_UNITS");
+  const bool sup = !small_k1_disabled() && !super_disabled() && !wide_gather_disabled() &&
+                   (pu ? pu[0] == '1' : (nchunks + 15) / 16 >= 8ull * ctx->num_cus);
+  // 16 chunks per unit for byte symbols; 8 (two workgroups per CU, whose
+  // loads and compute overlap) for wider ones
+  auto pass_waves = [&](uint32_t w) { return w == 1 ? 16u : 8u; };
+  const uint64_t units_max = (nchunks + 7) / 8;
+  const uint64_t tstride =
+      sup ? std::max((nchunks + 15) / 16 * (16 * kChunkSyms / 64), units_max * (8 * kChunkSyms / 64)) + 64 : stride;
+  const uint32_t wide_nstep = wide_gather_steps();
+  const uint64_t wide_wpl = (words_for(n) + 128ull * wide_nstep - 1) / (128ull * wide_nstep);
   auto carve = [&](arena& A, std::vector<void*>& v) {
     v.clear();
-    v.push_back(A.take<uint64_t>(std::max<size_t>(1, size_t(gmax - 1) * stride)));  // 0 temp
+    v.push_back(A.take<uint64_t>(std::max<size_t>(1, size_t(gmax - 1) * tstride)));  // 0 temp
     v.push_back(A.take<uint32_t>(sz_cnt));                       // 1 counts
     v.push_back(A.take<uint64_t>(sz_cnt));                       // 2 tnz scratch
     v.push_back(A.take<uint64_t>(std::max<size_t>(1, ntiles_max * nslot_max)));  // 3 tsum
@@ -2988,6 +3083,9 @@ void build_passes_device(pwmhip_ctx* ctx, const void* d_text, uint64_t n, uint32
     v.push_back(A.take<unsigned long long>(1));                  // 13 sym_or
     v.push_back(A.take<uint8_t>(P > 1 ? text_bytes : 1));        // 14 reorder buffer A
     v.push_back(A.take<uint8_t>(P > 2 ? text_bytes : 1));        // 15 reorder buffer B
+    v.push_back(A.take<uint64_t>(sup ? std::max<size_t>(1, units_max << gmax) : 1));  // 16 unit positions
+    v.push_back(A.take<uint32_t>(sup ? std::max<size_t>(1, nchunks << gmax) : 1));  // 17 chunk counts
+    v.push_back(A.take<uint32_t>(sup ? std::max<size_t>(1, size_t(gmax) * wide_wpl) : 1));  // 18 wave firsts
   };
   std::vector<void*> b;
   {
@@ -3024,32 +3122,63 @@ void build_passes_device(pwmhip_ctx* ctx, const void* d_text, uint64_t n, uint32
     k1.out_level0 = ws->d_words + ws->level_offset[l0];
     k1.temp_levels = U64(0);
     k1.temp_payload = U64(0);
-    k1.temp_stride = stride;
+    k1.temp_stride = tstride;
     k1.counts = counts;
     k1.nslot = nslot;
     k1.zeros_out = reinterpret_cast<unsigned long long*>(U64(12) + l0);
     k1.sym_or = static_cast<unsigned long long*>(b[13]);
     k1.nchunks = nchunks;
     k1.lut = lut;
-    dispatch_small(ctx, k1, !last);
+    const uint32_t pw = pass_waves(cw);  // chunks per unit of this pass
+    const uint64_t units = (nchunks + pw - 1) / pw;
+    const uint64_t nu = sup ? units : nchunks;  // run units
+    const uint64_t usyms = sup ? pw * kChunkSyms : kChunkSyms;
+    if (sup) {
+      k1.chunk_sc = static_cast<uint32_t*>(b[17]);
+      if (pw == 16) launch_pass_units<16>(ctx, k1, g, !last);
+      else launch_pass_units<8>(ctx, k1, g, !last);
+    } else {
+      dispatch_small(ctx, k1, !last);
+    }
     if (i == 0) PWM_HIP_TRY(hipEventRecord(ctx->ev[1], s));
     if (nslot) {
-      const uint64_t ntiles = (nchunks + kTile - 1) / kTile;
+      const uint64_t ntiles = (nu + kTile - 1) / kTile;
       hipLaunchKernelGGL(k_tile_sums2, dim3(uint32_t(ntiles), nslot), dim3(256), 0, s, counts, U64(3), U64(2),
-                         nchunks, nslot, ntiles);
+                         nu, nslot, ntiles);
       hipLaunchKernelGGL(k_tile_scan, dim3(nslot), dim3(1024), 0, s, U64(3), U64(4), ntiles);
       hipLaunchKernelGGL(k_tile_scan, dim3(nslot), dim3(1024), 0, s, U64(2), U64(5), ntiles);
       const uint32_t jb = last ? g - 1 : g;  // sub-levels with bucket starts
       hipLaunchKernelGGL(k_bucket_starts, dim3(jb), dim3(256), 0, s, U64(4), U64(6), 0, (uint64_t*)nullptr);
       hipLaunchKernelGGL(k_bucket_starts, dim3(jb), dim3(256), 0, s, U64(5), U64(7), 0, U64(8));
       hipLaunchKernelGGL(k_apply_runs, dim3(uint32_t(ntiles), nslot), dim3(1024), 0, s, counts, U64(3), U64(2),
-                         U64(6), U64(7), U64(9), U64(10), nchunks, nslot, ntiles, U64(11),
-                         last ? nslot : (1u << g) - 2u, kChunkSyms);
-      if (i == 0) PWM_HIP_TRY(hipEventRecord(ctx->ev[2], s));
+                         U64(6), U64(7), U64(9), U64(10), nu, nslot, ntiles, sup ? U64(16) : U64(11),
+                         last ? nslot : (1u << g) - 2u, usyms);
+      if (sup && !last)
+        hipLaunchKernelGGL(k_chunk_dstpos, dim3(uint32_t(units)), dim3(256), 0, s, U64(16),
+                           static_cast<const uint32_t*>(b[17]), U64(11), nchunks, 1u << g, pw);
+      if (sup) {
+        WideParams wp{};
+        wp.tab_dst = U64(9);
+        wp.tab_src = U64(10);
+        wp.level_ne = U64(8);
+        wp.first = static_cast<uint32_t*>(b[18]);
+        wp.src = U64(0);
+        wp.src_stride = tstride;
+        for (uint32_t j = 1; j < g; ++j) wp.dst[j] = ws->d_words + ws->level_offset[l0 + j];
+        wp.units = nu;
+        wp.n = n;
+        wp.wpl = wide_wpl;
+        wp.nstep = wide_nstep;
+        hipLaunchKernelGGL(k_wave_starts, dim3(uint32_t((wide_wpl + 255) / 256), g - 1), dim3(256), 0, s, wp);
+        if (i == 0) PWM_HIP_TRY(hipEventRecord(ctx->ev[2], s));
+        hipLaunchKernelGGL(k_gather_wide, dim3(uint32_t((wide_wpl + 3) / 4), g - 1), dim3(256), 0, s, wp);
+      } else if (i == 0) {
+        PWM_HIP_TRY(hipEventRecord(ctx->ev[2], s));
+      }
       const uint64_t W = words_for(n);
       const uint32_t nwin = gather_windows(W, ctx->num_cus);
       const uint32_t gblocks = uint32_t((W + 256ull * nwin - 1) / (256ull * nwin));
-      for (uint32_t j = 1; j < g; ++j) {
+      for (uint32_t j = 1; !sup && j < g; ++j) {
         GatherParams gp{};
         gp.nwin = nwin;
         const uint64_t toff = uint64_t((1u << j) - 2) * nchunks;
@@ -3070,9 +3199,9 @@ void build_passes_device(pwmhip_ctx* ctx, const void* d_text, uint64_t n, uint32
         sp.n = n;
         sp.sh = R - g;
         sp.dstpos = U64(11);
-        sp.counts = counts;
-        sp.nslot = nslot;
-        sp.slot0 = (1u << g) - 2u;
+        sp.counts = sup ? static_cast<const uint32_t*>(b[17]) : counts;
+        sp.nslot = sup ? (1u << g) : nslot;
+        sp.slot0 = sup ? 0u : (1u << g) - 2u;
         sp.nchunks = nchunks;
         const uint32_t wout = width_for_bits(R - g);
         launch_scatter(s, cw, wout, g, sp);

@@ -140,3 +140,32 @@ def test_zipf_sparse_buckets_vs_oracle(w, L):
         if not tree:
             assert np.array_equal(ws.zeros(), o.zeros)
         ws.free()
+
+
+@pytest.mark.gpu
+@pytest.mark.parametrize("w,L", [(1, 5), (1, 8), (2, 9), (2, 16), (4, 13), (4, 20)])
+def test_passes_over_units_vs_oracle(w, L):
+    """Builds of more than 4 levels with every pass over 16-chunk (byte
+    symbols) or 8-chunk (wider symbols) units -- the path large texts take --
+    at sizes around the unit boundaries, uniform and heavy-tailed symbols."""
+    import os
+    rng = np.random.default_rng(7000 + 10 * w + L)
+    os.environ["PWMHIP_PASS_UNITS"] = "1"
+    try:
+        for n in [1, 4097, 32768, 65535, 65536, 65537, 3 * 65536 + 4099, 300001]:
+            for kind in ("uniform", "skew"):
+                t = _random_case(rng, w, L, n, kind)
+                for tree in (False, True):
+                    o = O.naive(t, L, tree)
+                    ws = pwm_amd.build(t, L, "pps", tree, check_range=True)
+                    _assert_same(ws, o.levels, ("units", w, L, n, kind, tree))
+                    if not tree:
+                        assert np.array_equal(ws.zeros(), o.zeros)
+                    ws.free()
+        t = pwm_amd.generate("zipf", 0x5EED + L, 1 << 20, w, L)
+        o = O.naive(t, L, False)
+        ws = pwm_amd.build(t, L, "pps", False, check_range=True)
+        _assert_same(ws, o.levels, ("units zipf", w, L))
+        ws.free()
+    finally:
+        del os.environ["PWMHIP_PASS_UNITS"]
