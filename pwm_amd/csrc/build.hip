@@ -2311,6 +2311,7 @@ __global__ __launch_bounds__(256) void k_gather(GatherParams p) {
 // Batch: waves are assigned to texts by wave_base (per text), level j.
 struct BatchGatherParams {
   const uint64_t* wave_base;   // ntexts + 1
+  const uint32_t* wave_text;   // text of every gather wave
   const uint64_t* chunk_base;  // ntexts + 1
   const uint64_t* text_n;
   const uint64_t* out0;        // word offset of each text's level 0
@@ -2330,13 +2331,7 @@ __global__ __launch_bounds__(256) void k_gather_batch(BatchGatherParams p) {
   __shared__ unsigned long long stage[4][64];
   const uint64_t wv = uint64_t(blockIdx.x) * 4 + (threadIdx.x >> 6);
   if (wv >= p.wave_base[p.ntexts]) return;
-  uint32_t lo = 0, hi = p.ntexts;  // wave_base[lo] <= wv < wave_base[hi]
-  while (hi - lo > 1) {
-    const uint32_t mid = (lo + hi) >> 1;
-    if (p.wave_base[mid] <= wv) lo = mid;
-    else hi = mid;
-  }
-  const uint32_t t = lo;
+  const uint32_t t = p.wave_text[wv];
   const uint64_t W0 = (wv - p.wave_base[t]) * 64 * p.nwin;
   const uint64_t toff = uint64_t((1u << p.j) - 2) * p.nchunks + p.chunk_base[t] * (1ull << p.j);
   gather_range(p.tab_dst + toff, p.tab_src + toff, p.level_ne[uint64_t(t) * 8 + p.j],
@@ -2345,20 +2340,15 @@ __global__ __launch_bounds__(256) void k_gather_batch(BatchGatherParams p) {
 }
 
 // Batch stitch over super-chunk runs: waves belong to texts by wave_base
-// (steps of 128 words, nwin steps per wave); a wave finds its first run in
-// its text's table slice with a 64-way ballot search.
+// (steps of 128 words, nwin steps per wave; the text of every wave from a
+// table); a wave finds its first run in its text's table slice with a 64-way
+// ballot search.
 __global__ __launch_bounds__(256) void k_gather_batch_wide(BatchGatherParams p) {
   __shared__ unsigned long long stage[4][64];
   const uint32_t lane = threadIdx.x & 63;
   const uint64_t wv = uint64_t(blockIdx.x) * 4 + (threadIdx.x >> 6);
   if (wv >= p.wave_base[p.ntexts]) return;  // whole waves; no barrier below
-  uint32_t lo_t = 0, hi_t = p.ntexts;  // wave_base[lo] <= wv < wave_base[hi]
-  while (hi_t - lo_t > 1) {
-    const uint32_t mid = (lo_t + hi_t) >> 1;
-    if (p.wave_base[mid] <= wv) lo_t = mid;
-    else hi_t = mid;
-  }
-  const uint32_t t = lo_t;
+  const uint32_t t = p.wave_text[wv];
   const uint64_t W0 = (wv - p.wave_base[t]) * 128 * p.nwin;
   const uint64_t toff = uint64_t((1u << p.j) - 2) * p.nchunks + p.chunk_base[t] * (1ull << p.j);
   const uint64_t* td = p.tab_dst + toff;
@@ -2684,6 +2674,13 @@ __global__ void k_fill_u64(uint64_t* p, uint64_t n, uint64_t v) {
 __global__ void k_fill_u32(unsigned int* p, uint32_t n, uint32_t v) {
   const uint32_t i = blockIdx.x * blockDim.x + threadIdx.x;
   if (i < n) p[i] = v;
+}
+
+// The text of every batch gather wave: waves [wave_base[t], wave_base[t+1]).
+__global__ __launch_bounds__(256) void k_wave_text(const uint64_t* wave_base, uint32_t ntexts, uint32_t* wave_text) {
+  const uint32_t t = blockIdx.x * blockDim.x + threadIdx.x;
+  if (t >= ntexts) return;
+  for (uint64_t w = wave_base[t]; w < wave_base[t + 1]; ++w) wave_text[w] = t;
 }
 
 // Positions of every chunk's buckets in the order after a super-chunk pass:
@@ -3643,6 +3640,7 @@ void build_batch_device(pwmhip_ctx* ctx, const void* const* d_texts, const uint6
   const uint64_t tstride = C * (unit_syms / 64) + 64;
   const uint64_t ntiles = (C + kTile - 1) / kTile;
   const size_t sz_cnt = std::max<size_t>(1, C * nslot);
+  const uint64_t wv_max = waves_for_nwin(1);  // (the most gather waves of any geometry)
   arena sizing;
   sizing.cap = ~size_t(0) >> 1;
   auto carve = [&](arena& A, std::vector<void*>& v) {
@@ -3674,6 +3672,7 @@ void build_batch_device(pwmhip_ctx* ctx, const void* const* d_texts, const uint6
     for (int k = 0; k < 4; ++k)                            // 24..27 text bounds
       v.push_back(A.take<uint64_t>(std::max<size_t>(1, T * nslot)));
     v.push_back(A.take<UnitMeta>(sunits ? std::max<size_t>(1, C) : 1));  // 28 unit metadata
+    v.push_back(A.take<uint32_t>(std::max<uint64_t>(1, wv_max)));         // 29 text of every gather wave
   };
   std::vector<void*> b;
   carve(sizing, b);
@@ -3777,9 +3776,12 @@ void build_batch_device(pwmhip_ctx* ctx, const void* const* d_texts, const uint6
                          U64(10), ctext, U64(0), U64(24), U64(25), U64(16), U64(17), U64(19), U64(20), C, nslot,
                          ntiles, unit_syms);
       if (events) PWM_HIP_TRY(hipEventRecord(ctx->ev[2], s));
+      hipLaunchKernelGGL(k_wave_text, dim3(uint32_t((T + 255) / 256)), dim3(256), 0, s, U64(5), uint32_t(T),
+                         static_cast<uint32_t*>(b[29]));
       for (uint32_t j = 1; j <= jmax; ++j) {
         BatchGatherParams bp{};
         bp.wave_base = U64(5);
+        bp.wave_text = static_cast<const uint32_t*>(b[29]);
         bp.chunk_base = U64(0);
         bp.text_n = U64(2);
         bp.out0 = U64(3);
