@@ -283,6 +283,25 @@ struct HStitch {
   uint32_t temp_l0;            // level l of temp at temp + (l - temp_l0) * temp_stride
 };
 
+// Words of levels >= 1 that hold a run's first or last bit (shared with the
+// neighbouring run, or a level's last word): cleared before the run stitch,
+// which writes every other word whole and ORs these.  Enough when the nodes
+// of every level tile it without gaps (huff_naive's layout).  One thread per
+// (unit, slot).
+__global__ __launch_bounds__(256) void k_huff_clear_edges(HStitch p, uint64_t units) {
+  const uint64_t i = uint64_t(blockIdx.x) * 256 + threadIdx.x;
+  if (i >= units * p.nslot) return;
+  const uint64_t c = i / p.nslot;
+  const uint32_t s = uint32_t(i % p.nslot);
+  const uint64_t len = p.counts[c * p.nslot + s];
+  if (!len) return;
+  const uint32_t l = p.slot_level[s];
+  const uint64_t d0 = p.gstart[s] + p.pref[c * p.nslot + s], d1 = d0 + len;
+  uint64_t* dst = p.dst[l];
+  if (d0 & 63) dst[d0 >> 6] = 0;
+  if (d1 & 63) dst[d1 >> 6] = 0;
+}
+
 // One block per unit, one thread per (run, destination word), runs looped.
 __global__ __launch_bounds__(256) void k_huff_stitch(HStitch p) {
   const uint64_t c = blockIdx.x;
@@ -485,10 +504,13 @@ void build_huffman_super(pwmhip_ctx* ctx, const uint8_t* d_text, uint64_t n, con
   PWM_HIP_TRY(hipMemcpyAsync(d_level_slot0, level_slot0.data(), (L + 1) * 4, hipMemcpyHostToDevice, st));
   PWM_HIP_TRY(hipMemcpyAsync(d_dst, dst.data(), L * sizeof(uint64_t*), hipMemcpyHostToDevice, st));
   PWM_HIP_TRY(hipMemcpyAsync(d_lbits, bits.data(), L * 8, hipMemcpyHostToDevice, st));
-  // runs of levels >= 1 meet inside words (and huff_pc's node borders can
-  // leave gaps): the levels start cleared
-  for (uint64_t l = 1; l < L; ++l)
-    PWM_HIP_TRY(hipMemsetAsync(dst[l], 0, words_for(bits[l]) * 8, st));
+  // runs of levels >= 1 meet inside words: with huff_naive's layout the
+  // words holding run ends are cleared after the scan (k_huff_clear_edges);
+  // huff_pc's node borders can leave gaps, so there the levels start cleared
+  const bool edges_only = ws->is_huffman == 2;
+  if (!edges_only)
+    for (uint64_t l = 1; l < L; ++l)
+      PWM_HIP_TRY(hipMemsetAsync(dst[l], 0, words_for(bits[l]) * 8, st));
 
   PWM_HIP_TRY(hipEventRecord(ctx->ev[0], st));
   launch_huff_super(ctx, hp);
@@ -511,6 +533,9 @@ void build_huffman_super(pwmhip_ctx* ctx, const uint8_t* d_text, uint64_t n, con
     hs.all_atomic = 0;
     hs.unit_syms = kSuperChunkSyms;
     hs.temp_l0 = 1;
+    if (edges_only)
+      hipLaunchKernelGGL(k_huff_clear_edges, dim3(uint32_t((units * nslot + 255) / 256)), dim3(256), 0, st, hs,
+                         units);
     hipLaunchKernelGGL(k_huff_stitch, dim3(uint32_t(units)), dim3(256), 0, st, hs);
     PWM_HIP_TRY(hipGetLastError());
   } else {
