@@ -139,6 +139,72 @@ __global__ __launch_bounds__(256) void k_hist_bytes(const uint8_t* text, uint64_
     if (lh[i]) atomicAdd(&hist[i], (unsigned long long)lh[i]);
 }
 
+// Byte histogram for texts whose values are mostly below 8 (DNA-like): a
+// lane adds 1 << 8 v into one 64-bit register of eight 8-bit counters
+// (about 4 instructions a byte against some 10 for the candidate tests),
+// emptied into 32-bit totals every 240 bytes; words with a byte of 8 or more
+// count byte by byte, those bytes in LDS.
+__global__ __launch_bounds__(256) void k_hist_small(const uint8_t* text, uint64_t n, unsigned long long* hist) {
+  __shared__ unsigned int lh[256];
+  for (uint32_t i = threadIdx.x; i < 256; i += blockDim.x) lh[i] = 0;
+  __syncthreads();
+  uint32_t tot[8] = {0, 0, 0, 0, 0, 0, 0, 0};
+  uint64_t acc = 0;
+  uint32_t since = 0;  // bytes added to acc since it was emptied
+  auto empty = [&]() {
+#pragma unroll
+    for (int v = 0; v < 8; ++v) tot[v] += uint32_t(acc >> (8 * v)) & 0xffu;
+    acc = 0;
+    since = 0;
+  };
+  const bool aligned = (reinterpret_cast<uintptr_t>(text) & 15) == 0;
+  const uint64_t tid = uint64_t(blockIdx.x) * blockDim.x + threadIdx.x;
+  const uint64_t nthr = uint64_t(gridDim.x) * blockDim.x;
+  for (uint64_t i0 = tid * 16; i0 < n; i0 += nthr * 16) {
+    uint32_t w[4] = {0, 0, 0, 0}, vmask[4] = {0, 0, 0, 0};
+    if (i0 + 16 <= n && aligned) {
+      const uint4 v = *reinterpret_cast<const uint4*>(text + i0);
+      w[0] = v.x; w[1] = v.y; w[2] = v.z; w[3] = v.w;
+      vmask[0] = vmask[1] = vmask[2] = vmask[3] = 0x80808080u;
+    } else {
+      const uint32_t nv = uint32_t(min<uint64_t>(16, n - i0));
+      for (uint32_t k = 0; k < nv; ++k) {
+        w[k >> 2] |= uint32_t(text[i0 + k]) << (8 * (k & 3));
+        vmask[k >> 2] |= 0x80u << (8 * (k & 3));
+      }
+    }
+#pragma unroll
+    for (int q = 0; q < 4; ++q) {
+      if (vmask[q] == 0x80808080u && (w[q] & 0xf8f8f8f8u) == 0) {
+        acc += 1ull << (8 * (w[q] & 7u));
+        acc += 1ull << (8 * ((w[q] >> 8) & 7u));
+        acc += 1ull << (8 * ((w[q] >> 16) & 7u));
+        acc += 1ull << (8 * (w[q] >> 24));
+      } else {
+        for (int k = 0; k < 4; ++k) {
+          if (!((vmask[q] >> (8 * k + 7)) & 1)) continue;
+          const uint32_t c = (w[q] >> (8 * k)) & 0xffu;
+          if (c < 8) acc += 1ull << (8 * c);
+          else atomicAdd(&lh[c], 1u);
+        }
+      }
+    }
+    since += 16;
+    if (since >= 240) empty();  // (no counter above 255)
+  }
+  empty();
+#pragma unroll
+  for (int v = 0; v < 8; ++v) {
+    uint32_t t = tot[v];
+#pragma unroll
+    for (int d = 32; d >= 1; d >>= 1) t += __shfl_xor(t, d, 64);
+    if ((threadIdx.x & 63) == 0 && t) atomicAdd(&lh[v], t);
+  }
+  __syncthreads();
+  for (uint32_t i = threadIdx.x; i < 256; i += blockDim.x)
+    if (lh[i]) atomicAdd(&hist[i], (unsigned long long)lh[i]);
+}
+
 struct HParams {
   const void* text;
   uint32_t width;
@@ -592,7 +658,15 @@ void build_huffman_device(pwmhip_ctx* ctx, const void* d_text, uint64_t n, uint3
           hc.rep[k] = ord[k] * 0x01010101u;
           hc.used = k + 1;
         }
-        hipLaunchKernelGGL(k_hist_bytes, dim3(grid), dim3(256), 0, st, (const uint8_t*)d_text, n, hc, d_hist);
+        uint64_t below8 = 0, all = 0;  // (the sample decides the kernel; both count exactly)
+        for (uint32_t c = 0; c < 256; ++c) {
+          all += sh[c];
+          if (c < 8) below8 += sh[c];
+        }
+        if (below8 * 1000 >= all * 999)
+          hipLaunchKernelGGL(k_hist_small, dim3(grid), dim3(256), 0, st, (const uint8_t*)d_text, n, d_hist);
+        else
+          hipLaunchKernelGGL(k_hist_bytes, dim3(grid), dim3(256), 0, st, (const uint8_t*)d_text, n, hc, d_hist);
         break;
       }
       case 2: hipLaunchKernelGGL(k_hist<uint16_t>, dim3(grid), dim3(256), 0, st, (const uint16_t*)d_text, n, nsym, d_hist, first_arg, d_max); break;

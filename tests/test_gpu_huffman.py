@@ -107,3 +107,23 @@ def test_huffman_small_codes_vs_oracle(values):
                 _same(ws, o.levels, o.bits, (values, n, tree, algo))
                 assert np.array_equal(ws.zeros(), o.zeros), (values, n, tree, algo)
                 ws.free()
+
+
+@pytest.mark.gpu
+def test_huffman_histogram_small_alphabet_with_rare_outliers():
+    """Byte texts almost all below 8 take the packed-counter histogram; rare
+    larger bytes (below the sample's one-in-a-thousand bar, some past the
+    1 MiB sample) are counted byte by byte.  Unaligned views too."""
+    rng = np.random.default_rng(0xB17)
+    n = 3 * (1 << 20) + 12345
+    base = rng.integers(0, 5, n + 7, dtype=np.uint8)
+    pos = rng.choice(n + 7, 40, replace=False)
+    base[pos] = rng.integers(8, 256, pos.size, dtype=np.uint8)
+    base[(1 << 20) + 999] = 7
+    for off in (0, 3):
+        t = base[off:off + n]
+        o = O.huff_naive(t, False)
+        ws = pwm_amd.build(t, 0, "huff_naive", False)
+        _same(ws, o.levels, o.bits, ("outliers", off))
+        assert np.array_equal(ws.zeros(), o.zeros), off
+        ws.free()
