@@ -1163,13 +1163,14 @@ struct UnitMeta {
   uint64_t t;     // text
 };
 
-// One thread per text: the metadata of its units, and unit -> text.
-__global__ void k_unit_meta(const uint64_t* base, const uint64_t* text, const uint64_t* n, const uint64_t* out0,
-                            uint32_t ntexts, uint32_t width, uint64_t unit_syms, UnitMeta* meta,
-                            uint32_t* unit_text) {
-  const uint32_t t = blockIdx.x * blockDim.x + threadIdx.x;
+// One wave per text (lanes over its units): the metadata of its units, and
+// unit -> text.
+__global__ __launch_bounds__(256) void k_unit_meta(const uint64_t* base, const uint64_t* text, const uint64_t* n,
+                                                   const uint64_t* out0, uint32_t ntexts, uint32_t width,
+                                                   uint64_t unit_syms, UnitMeta* meta, uint32_t* unit_text) {
+  const uint32_t t = blockIdx.x * 4 + (threadIdx.x >> 6);
   if (t >= ntexts) return;
-  for (uint64_t u = base[t]; u < base[t + 1]; ++u) {
+  for (uint64_t u = base[t] + (threadIdx.x & 63); u < base[t + 1]; u += 64) {
     const uint64_t lu = u - base[t];
     UnitMeta m;
     m.text = text[t] + lu * unit_syms * width;
@@ -3748,7 +3749,7 @@ void build_batch_device(pwmhip_ctx* ctx, const void* const* d_texts, const uint6
     k1.grab = tu.grab;
     k1.b_chunk_text = static_cast<uint32_t*>(b[23]);
     if (sunits) {
-      hipLaunchKernelGGL(k_unit_meta, dim3(uint32_t((T + 255) / 256)), dim3(256), 0, s, U64(0), U64(1), U64(2),
+      hipLaunchKernelGGL(k_unit_meta, dim3(uint32_t((T + 3) / 4)), dim3(256), 0, s, U64(0), U64(1), U64(2),
                          U64(3), uint32_t(T), width, unit_syms, static_cast<UnitMeta*>(b[28]),
                          static_cast<uint32_t*>(b[23]));
       launch_super_batch(ctx, k1, L, static_cast<const UnitMeta*>(b[28]));
