@@ -351,6 +351,45 @@ __device__ __forceinline__ void load_planes_text(const K1Params& p, const void* 
       return;
     }
   }
+  if constexpr (sizeof(T) == 1) {
+    // bytes (partial chunks, unaligned texts): four groups of 16 bytes, each
+    // group's loads issued together (one 16-byte load when aligned), so a
+    // lane waits for 4 round trips rather than 16
+    const uint8_t* lp = static_cast<const uint8_t*>(text) + e0;
+    const bool a16 = (reinterpret_cast<uintptr_t>(lp) & 15) == 0;
+#pragma unroll 1
+    for (int g = 0; g < 4; ++g) {
+      uint32_t w[4];
+      if (cnt >= 16 * g + 16) {
+        if (a16) {
+          const uint4 x = *gmem(reinterpret_cast<const uint4*>(lp + 16 * g));
+          w[0] = x.x; w[1] = x.y; w[2] = x.z; w[3] = x.w;
+        } else {
+#pragma unroll
+          for (int i = 0; i < 4; ++i) w[i] = *gmem(reinterpret_cast<const uint32_t*>(lp + 16 * g + 4 * i));
+        }
+      } else {
+#pragma unroll
+        for (int i = 0; i < 4; ++i) {
+          const int q = 4 * g + i;
+          if (cnt >= 4 * q + 4) {
+            w[i] = *gmem(reinterpret_cast<const uint32_t*>(lp + 4 * q));
+          } else {
+            uint32_t x = 0;
+            for (int e = 0; e < 4; ++e)
+              if (4 * q + e < cnt) x |= uint32_t(tx[e0 + 4 * q + e]) << (8 * e);
+            w[i] = x;
+          }
+        }
+      }
+#pragma unroll
+      for (int i = 0; i < 4; ++i) {
+        const uint64_t v[4] = {w[i] & 0xffu, (w[i] >> 8) & 0xffu, (w[i] >> 16) & 0xffu, w[i] >> 24};
+        planes4(v, 4 * g + i);
+      }
+    }
+    return;
+  }
 #pragma unroll 1
   for (int q = 0; q < 16; ++q) {
     uint64_t v[4];
