@@ -2309,6 +2309,7 @@ struct WideParams {
   uint64_t* dst[kMaxGroup];  // destination level of sub-level j (index j)
   uint64_t units, n, wpl;    // run units, symbols, waves per level
   uint32_t nstep;
+  uint64_t nbits[kMaxGroup]; // bits of sub-level j when its own (0: n)
 };
 
 // First table entry of every wave of the wide stitch: the last entry whose
@@ -2320,8 +2321,9 @@ __global__ __launch_bounds__(256) void k_wave_starts(WideParams p) {
   const uint64_t X = k * 128ull * p.nstep * 64;
   const uint64_t* td = p.tab_dst + uint64_t((1u << j) - 2) * p.units;
   const uint64_t NE = p.level_ne[j];
+  const uint64_t nj = p.nbits[j] ? p.nbits[j] : p.n;
   uint64_t lo = 0, hi = NE;  // td[lo] <= X < td[hi] (td[0] == 0)
-  if (X < p.n && NE > 0) {
+  if (X < nj && NE > 0) {
     while (hi - lo > 1) {
       const uint64_t mid = (lo + hi) >> 1;
       if (td[mid] <= X) lo = mid;
@@ -2338,7 +2340,8 @@ __global__ __launch_bounds__(256) void k_gather_wide(WideParams p) {
   if (wave >= p.wpl) return;  // whole waves; no barrier below
   const uint64_t toff = uint64_t((1u << j) - 2) * p.units;
   gather_range_wide(p.tab_dst + toff, p.tab_src + toff, p.level_ne[j], p.src + (j - 1) * p.src_stride, p.dst[j],
-                    p.n, wave * 128 * p.nstep, p.nstep, p.first[(j - 1) * p.wpl + wave], stage[threadIdx.x >> 6]);
+                    p.nbits[j] ? p.nbits[j] : p.n, wave * 128 * p.nstep, p.nstep,
+                    p.first[(j - 1) * p.wpl + wave], stage[threadIdx.x >> 6]);
 }
 
 __global__ __launch_bounds__(256) void k_gather(GatherParams p) {
@@ -2867,6 +2870,37 @@ void launch_super(pwmhip_ctx* ctx, const K1Params& prm) {
 }
 
 }  // namespace
+
+uint64_t wide_stitch_first_words(uint64_t max_bits, uint32_t nsub) {
+  const uint32_t nstep = wide_gather_steps();
+  const uint64_t wpl = (words_for(max_bits) + 128ull * nstep - 1) / (128ull * nstep);
+  return std::max<uint64_t>(1, wpl * nsub);
+}
+
+void launch_wide_stitch(pwmhip_ctx* ctx, const WideStitch& w) {
+  if (w.nsub == 0) return;
+  WideParams wp{};
+  wp.tab_dst = w.tab_dst;
+  wp.tab_src = w.tab_src;
+  wp.level_ne = w.level_ne;
+  wp.first = w.first;
+  wp.src = w.src;
+  wp.src_stride = w.src_stride;
+  uint64_t maxb = 0;
+  for (uint32_t j = 1; j <= w.nsub; ++j) {
+    wp.dst[j] = w.dst[j];
+    wp.nbits[j] = std::max<uint64_t>(w.nbits[j], 1);
+    maxb = std::max(maxb, w.nbits[j]);
+  }
+  wp.units = w.units;
+  wp.n = maxb;
+  wp.nstep = wide_gather_steps();
+  wp.wpl = (words_for(maxb) + 128ull * wp.nstep - 1) / (128ull * wp.nstep);
+  hipStream_t s = ctx->stream;
+  hipLaunchKernelGGL(k_wave_starts, dim3(uint32_t((wp.wpl + 255) / 256), w.nsub), dim3(256), 0, s, wp);
+  hipLaunchKernelGGL(k_gather_wide, dim3(uint32_t((wp.wpl + 3) / 4), w.nsub), dim3(256), 0, s, wp);
+  PWM_HIP_TRY(hipGetLastError());
+}
 
 void launch_huff_super(pwmhip_ctx* ctx, const HuffSuperParams& hp) {
   static_assert(kSuperChunkSyms == kSuperSyms, "one super-chunk size");

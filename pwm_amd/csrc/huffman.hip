@@ -349,6 +349,15 @@ struct HStitch {
   uint32_t temp_l0;            // level l of temp at temp + (l - temp_l0) * temp_stride
 };
 
+// PWMHIP_HUFF_RUN_STITCH=1: the run-major stitch for huff_naive layouts too.
+bool wide_gather_disabled_huff() {
+  static const bool off = [] {
+    const char* e = std::getenv("PWMHIP_HUFF_RUN_STITCH");
+    return e && e[0] == '1';
+  }();
+  return off;
+}
+
 // Words of levels >= 1 that hold a run's first or last bit (shared with the
 // neighbouring run, or a level's last word): cleared before the run stitch,
 // which writes every other word whole and ORs these.  Enough when the nodes
@@ -366,6 +375,40 @@ __global__ __launch_bounds__(256) void k_huff_clear_edges(HStitch p, uint64_t un
   uint64_t* dst = p.dst[l];
   if (d0 & 63) dst[d0 >> 6] = 0;
   if (d1 & 63) dst[d1 >> 6] = 0;
+}
+
+// Run tables of a Huffman-shaped super-chunk build for the wide stitch
+// (huff_naive's layout: a level's nodes, numbered in destination order, tile
+// it).  Per-level bases: node s's first entry among its level's entries, and
+// the entries of every level.
+__global__ void k_huff_run_bases(const uint64_t* nztot, const uint32_t* slot_level, uint32_t nslot,
+                                 uint64_t* nebase, uint64_t* level_ne) {
+  if (threadIdx.x != 0 || blockIdx.x != 0) return;
+  for (uint32_t j = 0; j < kMaxGroup; ++j) level_ne[j] = 0;
+  for (uint32_t s = 0; s < nslot; ++s) {
+    const uint32_t l = slot_level[s];
+    nebase[s] = level_ne[l];
+    level_ne[l] += nztot[s];
+  }
+}
+
+// One thread per (unit, node) with symbols: its entry (destination bit,
+// source bit in the unit-local level) at level l's base + node base + its
+// rank among the node's non-empty units.
+__global__ __launch_bounds__(256) void k_huff_runs(HStitch p, uint64_t units, const uint64_t* nzpref,
+                                                   const uint64_t* nebase, uint64_t* tab_dst, uint64_t* tab_src) {
+  const uint64_t i = uint64_t(blockIdx.x) * 256 + threadIdx.x;
+  if (i >= units * p.nslot) return;
+  const uint64_t c = i / p.nslot;
+  const uint32_t s = uint32_t(i % p.nslot);
+  const uint32_t len = p.counts[c * p.nslot + s];
+  if (!len) return;
+  const uint32_t l = p.slot_level[s];
+  uint64_t loc = 0;  // the node's start in the unit's level-l bits: earlier nodes of the level
+  for (uint32_t t = p.level_slot0[l]; t < s; ++t) loc += p.counts[c * p.nslot + t];
+  const uint64_t e = uint64_t((1u << l) - 2) * units + nebase[s] + nzpref[c * p.nslot + s];
+  tab_dst[e] = p.gstart[s] + p.pref[c * p.nslot + s];
+  tab_src[e] = c * p.unit_syms + loc;
 }
 
 // One block per unit, one thread per (run, destination word), runs looped.
@@ -524,6 +567,16 @@ void build_huffman_super(pwmhip_ctx* ctx, const uint8_t* d_text, uint64_t n, con
   add(256); add((L - 1) * tstride * 8); add(units * ns * 4); add(units * ns * 8); add(ns * 8 + 8);
   add(ns * 8 + 8); add(L * 8); add(L * 8); add(scan_scratch_words(units, nslot) * 8); add((L + 1) * 4);
   add(ns * 4 + 4);
+  // the wide stitch (huff_naive's layout): non-empty ranks and counts, node
+  // bases, entries per level, tables, every wave's first entry
+  const bool wide = ws->is_huffman == 2 && !wide_gather_disabled_huff();
+  uint64_t maxbits = 0;
+  for (uint64_t l = 1; l < L; ++l) maxbits = std::max(maxbits, bits[l]);
+  const uint64_t ntab = ((uint64_t(1) << L) - 2) * units;
+  if (wide) {
+    add(units * ns * 8); add(ns * 8 + 8); add(ns * 8 + 8); add(kMaxGroup * 8); add(ntab * 8); add(ntab * 8);
+    add(wide_stitch_first_words(maxbits, uint32_t(L - 1)) * 4); add(scan_scratch_words(units, nslot) * 8);
+  }
   ctx->scratch.reset();
   ctx->scratch.ensure(need);
   arena& A = ctx->scratch;
@@ -538,6 +591,19 @@ void build_huffman_super(pwmhip_ctx* ctx, const uint8_t* d_text, uint64_t n, con
   uint64_t* d_tiles = A.take<uint64_t>(scan_scratch_words(units, nslot));
   uint32_t* d_level_slot0 = A.take<uint32_t>(L + 1);
   uint32_t* d_slot_level = A.take<uint32_t>(ns + 1);
+  uint64_t *d_nzpref = nullptr, *d_nztot = nullptr, *d_nebase = nullptr, *d_lne = nullptr;
+  uint64_t *d_tdst = nullptr, *d_tsrc = nullptr, *d_tiles2 = nullptr;
+  uint32_t* d_first = nullptr;
+  if (wide) {
+    d_nzpref = A.take<uint64_t>(units * ns);
+    d_nztot = A.take<uint64_t>(ns + 1);
+    d_nebase = A.take<uint64_t>(ns + 1);
+    d_lne = A.take<uint64_t>(kMaxGroup);
+    d_tdst = A.take<uint64_t>(ntab);
+    d_tsrc = A.take<uint64_t>(ntab);
+    d_first = A.take<uint32_t>(wide_stitch_first_words(maxbits, uint32_t(L - 1)));
+    d_tiles2 = A.take<uint64_t>(scan_scratch_words(units, nslot));
+  }
 
   std::vector<uint8_t> code(256, 0);
   for (uint64_t s = 0; s < ct.len.size() && s < 256; ++s)
@@ -573,7 +639,7 @@ void build_huffman_super(pwmhip_ctx* ctx, const uint8_t* d_text, uint64_t n, con
   // runs of levels >= 1 meet inside words: with huff_naive's layout the
   // words holding run ends are cleared after the scan (k_huff_clear_edges);
   // huff_pc's node borders can leave gaps, so there the levels start cleared
-  const bool edges_only = ws->is_huffman == 2;
+  const bool edges_only = ws->is_huffman == 2;  // (with `wide`: nothing to clear)
   if (!edges_only)
     for (uint64_t l = 1; l < L; ++l)
       PWM_HIP_TRY(hipMemsetAsync(dst[l], 0, words_for(bits[l]) * 8, st));
@@ -599,10 +665,32 @@ void build_huffman_super(pwmhip_ctx* ctx, const uint8_t* d_text, uint64_t n, con
     hs.all_atomic = 0;
     hs.unit_syms = kSuperChunkSyms;
     hs.temp_l0 = 1;
-    if (edges_only)
-      hipLaunchKernelGGL(k_huff_clear_edges, dim3(uint32_t((units * nslot + 255) / 256)), dim3(256), 0, st, hs,
-                         units);
-    hipLaunchKernelGGL(k_huff_stitch, dim3(uint32_t(units)), dim3(256), 0, st, hs);
+    if (wide) {
+      // destination-major: every word of every level written once
+      launch_scan_slots(st, d_counts, d_nzpref, d_nztot, units, nslot, d_tiles2, true);
+      hipLaunchKernelGGL(k_huff_run_bases, dim3(1), dim3(64), 0, st, d_nztot, d_slot_level, nslot, d_nebase, d_lne);
+      hipLaunchKernelGGL(k_huff_runs, dim3(uint32_t((units * nslot + 255) / 256)), dim3(256), 0, st, hs, units,
+                         d_nzpref, d_nebase, d_tdst, d_tsrc);
+      WideStitch w{};
+      w.tab_dst = d_tdst;
+      w.tab_src = d_tsrc;
+      w.level_ne = d_lne;
+      w.first = d_first;
+      w.src = d_temp;
+      w.src_stride = tstride;
+      for (uint64_t l = 1; l < L; ++l) {
+        w.dst[l] = dst[l];
+        w.nbits[l] = bits[l];
+      }
+      w.units = units;
+      w.nsub = uint32_t(L - 1);
+      launch_wide_stitch(ctx, w);
+    } else {
+      if (edges_only)
+        hipLaunchKernelGGL(k_huff_clear_edges, dim3(uint32_t((units * nslot + 255) / 256)), dim3(256), 0, st, hs,
+                           units);
+      hipLaunchKernelGGL(k_huff_stitch, dim3(uint32_t(units)), dim3(256), 0, st, hs);
+    }
     PWM_HIP_TRY(hipGetLastError());
   } else {
     PWM_HIP_TRY(hipEventRecord(ctx->ev[2], st));

@@ -182,6 +182,27 @@ struct HuffSuperParams {
 void launch_huff_super(pwmhip_ctx* ctx, const HuffSuperParams& hp);
 constexpr uint64_t kSuperChunkSyms = uint64_t(16) * kChunkSyms;  // symbols per super-chunk
 
+// Destination-major stitch of sub-levels 1..nsub from unit runs (build.hip's
+// wide stitch), for callers with their own run tables: level j's entries
+// (destination bit, source bit) in destination order from
+// ((1 << j) - 2) * units, level_ne[j] of them; sub-level j's unit-local bits
+// at src + (j - 1) * src_stride; nbits[j] bits written to dst[j].  `first`
+// holds wide_stitch_first_words(max nbits, nsub) words.
+struct WideStitch {
+  const uint64_t* tab_dst;
+  const uint64_t* tab_src;
+  const uint64_t* level_ne;
+  uint32_t* first;
+  const uint64_t* src;
+  uint64_t src_stride;
+  uint64_t* dst[kMaxGroup];
+  uint64_t nbits[kMaxGroup];
+  uint64_t units;
+  uint32_t nsub;
+};
+void launch_wide_stitch(pwmhip_ctx* ctx, const WideStitch& w);
+uint64_t wide_stitch_first_words(uint64_t max_bits, uint32_t nsub);
+
 // Huffman-shaped construction (huffman.hip).
 void build_huffman_device(pwmhip_ctx* ctx, const void* d_text, uint64_t n, uint32_t width,
                           bool tree, bool pc_quirks, pwmhip_ws* ws);
