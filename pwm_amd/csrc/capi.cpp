@@ -171,7 +171,7 @@ int pwmhip_build(pwmhip_ctx* ctx, const void* text, uint64_t size, uint32_t word
       d_text = ctx->text_scratch.base;
     }
     out->text_size = size;
-    if (reuse_ptr && (huff || size == 0)) {
+    if (reuse_ptr && size == 0) {
       (void)hipFree(reuse_ptr);
       reuse_ptr = nullptr;
     }
@@ -182,7 +182,9 @@ int pwmhip_build(pwmhip_ctx* ctx, const void* text, uint64_t size, uint32_t word
       // 18-121) gives huff_naive's levels with zeros[levels-1] left at 0.
       const bool quirks = algo == PWMHIP_ALGO_HUFF_PC || algo == PWMHIP_ALGO_HUFF_PS ||
                           algo == PWMHIP_ALGO_HUFF_DD_PC;
-      pwm::build_huffman_device(ctx, d_text, size, word_width, tree, quirks, out);
+      uint64_t* r = reuse_ptr;
+      reuse_ptr = nullptr;  // (the Huffman build takes it over or frees it)
+      pwm::build_huffman_device(ctx, d_text, size, word_width, tree, quirks, out, r, reuse_bytes);
       if (algo == PWMHIP_ALGO_HUFF_PC_SS && out->n_zeros) out->zeros[out->n_zeros - 1] = 0;
       return;
     }

@@ -707,7 +707,13 @@ void build_huffman_super(pwmhip_ctx* ctx, const uint8_t* d_text, uint64_t n, con
 }
 
 void build_huffman_device(pwmhip_ctx* ctx, const void* d_text, uint64_t n, uint32_t width, bool tree,
-                          bool pc_quirks, pwmhip_ws* ws) {
+                          bool pc_quirks, pwmhip_ws* ws, uint64_t* reuse, uint64_t reuse_bytes) {
+  struct ReuseGuard {  // an earlier output not taken over is freed on every exit
+    uint64_t*& p;
+    ~ReuseGuard() {
+      if (p) (void)hipFree(p);
+    }
+  } reuse_guard{reuse};
   hipStream_t st = ctx->stream;
   ws->is_tree = tree;
   ws->is_huffman = 2;  // (an empty structure has the naive layout)
@@ -900,7 +906,9 @@ void build_huffman_device(pwmhip_ctx* ctx, const void* d_text, uint64_t n, uint3
 
   // 4. output allocation
   std::vector<uint64_t> bits(ct.level_sizes.begin(), ct.level_sizes.end());
-  alloc_structure(ws, L, bits.data());
+  uint64_t* earlier = reuse;
+  reuse = nullptr;  // alloc_structure takes the earlier output over or frees it
+  alloc_structure(ws, L, bits.data(), earlier, reuse_bytes);
   ws->n_zeros = tree ? 0 : L;
   for (uint64_t l = 0; l < L; ++l) ws->zeros[l] = tree ? 0 : zeros[l];
 

@@ -204,8 +204,11 @@ void launch_wide_stitch(pwmhip_ctx* ctx, const WideStitch& w);
 uint64_t wide_stitch_first_words(uint64_t max_bits, uint32_t nsub);
 
 // Huffman-shaped construction (huffman.hip).
+// `reuse`: an earlier output block of `reuse_bytes` bytes, taken over when
+// large enough, freed otherwise (also when the build fails).
 void build_huffman_device(pwmhip_ctx* ctx, const void* d_text, uint64_t n, uint32_t width,
-                          bool tree, bool pc_quirks, pwmhip_ws* ws);
+                          bool tree, bool pc_quirks, pwmhip_ws* ws, uint64_t* reuse = nullptr,
+                          uint64_t reuse_bytes = 0);
 
 // copy_bits on the device (bitops.hip).
 void copy_bits_device(hipStream_t s, uint64_t* dst, const uint64_t* src, uint64_t dst_off,

@@ -127,3 +127,19 @@ def test_huffman_histogram_small_alphabet_with_rare_outliers():
         _same(ws, o.levels, o.bits, ("outliers", off))
         assert np.array_equal(ws.zeros(), o.zeros), off
         ws.free()
+
+
+@pytest.mark.gpu
+def test_huffman_reused_output():
+    """Huffman builds into an earlier structure's memory (large enough: taken
+    over; too small: replaced), results unchanged."""
+    rng = np.random.default_rng(0x5E)
+    ws = None
+    for n, k in ((300_000, 5), (20_000, 3), (600_000, 30)):
+        vals = rng.choice(200, k, replace=False).astype(np.uint8)
+        t = vals[rng.choice(k, n, p=rng.dirichlet(np.ones(k)))]
+        o = O.huff_naive(t, False)
+        ws = pwm_amd.build(t, 0, "huff_naive", False, reuse=ws)
+        _same(ws, o.levels, o.bits, ("reuse", n, k))
+        assert np.array_equal(ws.zeros(), o.zeros), (n, k)
+    ws.free()
