@@ -126,7 +126,9 @@ void wm_to_wt_device(pwmhip_ctx* ctx, const void* d_text, uint64_t n, uint32_t w
   uint64_t* d_tab = ctx->scratch.take<uint64_t>(tab);
   uint64_t* d_lv = ctx->scratch.take<uint64_t>(uint64_t(L - 1) * stride);
   PWM_HIP_TRY(hipMemsetAsync(d_hist, 0, K * 8, st));
-  const uint32_t grid = uint32_t(std::min<uint64_t>((n + 4095) / 4096, uint64_t(ctx->num_cus) * 8));
+  // (a grid-stride grid that is resident at once: the kernel's registers
+  // leave room for five waves a SIMD)
+  const uint32_t grid = uint32_t(std::min<uint64_t>((n + 4095) / 4096, uint64_t(ctx->num_cus) * 4));
   switch (width) {
     case 1: hipLaunchKernelGGL(k_leaf_hist<uint8_t>, dim3(grid), dim3(256), 0, st, (const uint8_t*)d_text, n, L, d_hist); break;
     case 2: hipLaunchKernelGGL(k_leaf_hist<uint16_t>, dim3(grid), dim3(256), 0, st, (const uint16_t*)d_text, n, L, d_hist); break;

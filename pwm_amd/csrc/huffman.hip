@@ -160,7 +160,86 @@ __global__ __launch_bounds__(256) void k_hist_small(const uint8_t* text, uint64_
   const bool aligned = (reinterpret_cast<uintptr_t>(text) & 15) == 0;
   const uint64_t tid = uint64_t(blockIdx.x) * blockDim.x + threadIdx.x;
   const uint64_t nthr = uint64_t(gridDim.x) * blockDim.x;
-  for (uint64_t i0 = tid * 16; i0 < n; i0 += nthr * 16) {
+  auto add4 = [&](uint32_t x) {  // four bytes below 8
+    acc += 1ull << (8 * (x & 7u));
+    acc += 1ull << (8 * ((x >> 8) & 7u));
+    acc += 1ull << (8 * ((x >> 16) & 7u));
+    acc += 1ull << (8 * (x >> 24));
+  };
+  // The bulk: 64 bytes a lane per step, four 16-byte loads issued together.
+  // 16 bytes whose values are all below 4 (or below 8) are packed into one
+  // word (two words) of 2-bit (3-bit) fields, and the lane counts the
+  // symbols with each set of value bits set: S0, S2, S01, S02, S12, S012
+  // and the set bits overall; the exact counts follow at the end by
+  // inclusion-exclusion.
+  uint32_t nb_sym = 0, s_all = 0, s0 = 0, s2 = 0, s01 = 0, s02 = 0, s12 = 0, s012 = 0;
+  auto count16 = [&](const uint4 v) {
+    const uint32_t o = v.x | v.y | v.z | v.w;
+    if ((o & 0xfcfcfcfcu) == 0) {
+      const uint32_t x = v.x | (v.y << 2) | (v.z << 4) | (v.w << 6);
+      s_all += __popc(x);
+      s0 += __popc(x & 0x55555555u);
+      s01 += __popc(x & (x >> 1) & 0x55555555u);
+      nb_sym += 16;
+    } else if ((o & 0xf8f8f8f8u) == 0) {
+#pragma unroll
+      for (int h = 0; h < 2; ++h) {
+        const uint32_t x = h ? (v.z | (v.w << 3)) : (v.x | (v.y << 3));
+        const uint32_t a = x & (x >> 1), b = x & (x >> 2);
+        s_all += __popc(x);
+        s0 += __popc(x & 0x09090909u);
+        s2 += __popc(x & 0x24242424u);
+        s01 += __popc(a & 0x09090909u);
+        s12 += __popc(a & 0x12121212u);
+        s02 += __popc(b & 0x09090909u);
+        s012 += __popc(a & b & 0x09090909u);
+      }
+      nb_sym += 16;
+    } else {
+      const uint32_t w4[4] = {v.x, v.y, v.z, v.w};
+#pragma unroll
+      for (int q = 0; q < 4; ++q) {
+        if ((w4[q] & 0xf8f8f8f8u) == 0) {
+          add4(w4[q]);
+        } else {
+          for (int e = 0; e < 4; ++e) {
+            const uint32_t c = (w4[q] >> (8 * e)) & 0xffu;
+            if (c < 8) acc += 1ull << (8 * c);
+            else atomicAdd(&lh[c], 1u);
+          }
+        }
+      }
+      since += 16;
+      if (since >= 240) empty();  // (no counter above 255)
+    }
+  };
+  uint64_t done = 0;  // bytes [0, done) handled here
+  if (aligned) {
+    const uint64_t nb = n / (nthr * 64) * (nthr * 64);
+    const uint4* tv = reinterpret_cast<const uint4*>(text);
+    // (each wave reads 4 KiB in a row per step)
+    const uint64_t wave = tid >> 6, nwaves = nthr >> 6, lane = tid & 63;
+    for (uint64_t blk = wave; blk * 4096 < nb; blk += nwaves) {
+      uint4 v[4];
+#pragma unroll
+      for (int k = 0; k < 4; ++k) v[k] = tv[blk * 256 + k * 64 + lane];
+#pragma unroll
+      for (int k = 0; k < 4; ++k) count16(v[k]);
+    }
+    done = nb;
+  }
+  {
+    const uint32_t s1 = s_all - s0 - s2;
+    tot[7] += s012;
+    tot[3] += s01 - s012;
+    tot[5] += s02 - s012;
+    tot[6] += s12 - s012;
+    tot[1] += s0 - s01 - s02 + s012;
+    tot[2] += s1 - s01 - s12 + s012;
+    tot[4] += s2 - s02 - s12 + s012;
+    tot[0] += nb_sym - s0 - s1 - s2 + s01 + s02 + s12 - s012;
+  }
+  for (uint64_t i0 = done + tid * 16; i0 < n; i0 += nthr * 16) {
     uint32_t w[4] = {0, 0, 0, 0}, vmask[4] = {0, 0, 0, 0};
     if (i0 + 16 <= n && aligned) {
       const uint4 v = *reinterpret_cast<const uint4*>(text + i0);
@@ -176,10 +255,7 @@ __global__ __launch_bounds__(256) void k_hist_small(const uint8_t* text, uint64_
 #pragma unroll
     for (int q = 0; q < 4; ++q) {
       if (vmask[q] == 0x80808080u && (w[q] & 0xf8f8f8f8u) == 0) {
-        acc += 1ull << (8 * (w[q] & 7u));
-        acc += 1ull << (8 * ((w[q] >> 8) & 7u));
-        acc += 1ull << (8 * ((w[q] >> 16) & 7u));
-        acc += 1ull << (8 * (w[q] >> 24));
+        add4(w[q]);
       } else {
         for (int k = 0; k < 4; ++k) {
           if (!((vmask[q] >> (8 * k + 7)) & 1)) continue;
@@ -758,7 +834,10 @@ void build_huffman_device(pwmhip_ctx* ctx, const void* d_text, uint64_t n, uint3
           if (c < 8) below8 += sh[c];
         }
         if (below8 * 1000 >= all * 999)
-          hipLaunchKernelGGL(k_hist_small, dim3(grid), dim3(256), 0, st, (const uint8_t*)d_text, n, d_hist);
+          // (four workgroups a CU: the kernel's registers leave room for
+          // seven waves a SIMD, and a grid-stride grid must be resident at once)
+          hipLaunchKernelGGL(k_hist_small, dim3(std::min<uint32_t>(grid, uint32_t(ctx->num_cus) * 4)), dim3(256), 0, st,
+                             (const uint8_t*)d_text, n, d_hist);
         else
           hipLaunchKernelGGL(k_hist_bytes, dim3(grid), dim3(256), 0, st, (const uint8_t*)d_text, n, hc, d_hist);
         break;
