@@ -139,11 +139,16 @@ __global__ __launch_bounds__(256) void k_hist_bytes(const uint8_t* text, uint64_
     if (lh[i]) atomicAdd(&hist[i], (unsigned long long)lh[i]);
 }
 
-// Byte histogram for texts whose values are mostly below 8 (DNA-like): a
-// lane adds 1 << 8 v into one 64-bit register of eight 8-bit counters
-// (about 4 instructions a byte against some 10 for the candidate tests),
-// emptied into 32-bit totals every 240 bytes; words with a byte of 8 or more
-// count byte by byte, those bytes in LDS.
+// Byte histogram for texts whose values are mostly below 8 (DNA-like).
+// 16 bytes all below 4 pack into one word of 2-bit fields (below 8: two
+// words of 3-bit fields, 6 bits a byte), and popcounts under masks give, per
+// lane, the number of symbols with every bit of each set of value bits set;
+// inclusion-exclusion turns these into exact counts at the end (about one
+// instruction a byte for 2-bit data, against some 10 for the candidate
+// tests).  16 bytes with a value of 8 or more go word by word into a 64-bit
+// register of eight 8-bit counters (emptied into 32-bit totals every 240
+// bytes), and bytes of 8 or more into LDS.  The unaligned head of the grid's
+// last step and an unaligned text go the 64-bit register way.
 __global__ __launch_bounds__(256) void k_hist_small(const uint8_t* text, uint64_t n, unsigned long long* hist) {
   __shared__ unsigned int lh[256];
   for (uint32_t i = threadIdx.x; i < 256; i += blockDim.x) lh[i] = 0;
