@@ -147,8 +147,8 @@ __global__ __launch_bounds__(256) void k_hist_bytes(const uint8_t* text, uint64_
 // instruction a byte for 2-bit data, against some 10 for the candidate
 // tests).  16 bytes with a value of 8 or more go word by word into a 64-bit
 // register of eight 8-bit counters (emptied into 32-bit totals every 240
-// bytes), and bytes of 8 or more into LDS.  The unaligned head of the grid's
-// last step and an unaligned text go the 64-bit register way.
+// bytes), and bytes of 8 or more into LDS.  The bytes after the last whole
+// grid step, and all of an unaligned text, go the 64-bit register way.
 __global__ __launch_bounds__(256) void k_hist_small(const uint8_t* text, uint64_t n, unsigned long long* hist) {
   __shared__ unsigned int lh[256];
   for (uint32_t i = threadIdx.x; i < 256; i += blockDim.x) lh[i] = 0;
