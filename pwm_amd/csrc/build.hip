@@ -1450,7 +1450,6 @@ __global__ __launch_bounds__(64 * kSuper) void k_huff_super(HuffSuperParams hp, 
   const uint32_t thi = uint32_t(cmap[4]) | (uint32_t(cmap[5]) << 8) | (uint32_t(cmap[6]) << 16) |
                        (uint32_t(cmap[7]) << 24);
   auto map4 = [&](uint32_t x) -> uint32_t {
-    if (hp.small_alpha) return __builtin_amdgcn_perm(thi, tlo, x);
     return uint32_t(cmap[x & 0xff]) | (uint32_t(cmap[(x >> 8) & 0xff]) << 8) |
            (uint32_t(cmap[(x >> 16) & 0xff]) << 16) | (uint32_t(cmap[x >> 24]) << 24);
   };
@@ -1458,9 +1457,12 @@ __global__ __launch_bounds__(64 * kSuper) void k_huff_super(HuffSuperParams hp, 
   const bool aligned = (reinterpret_cast<uintptr_t>(hp.text) & 15) == 0;
   auto full_chunk = [&](uint64_t c) { return aligned && c < hp.nchunks && (c + 1) * kChunkSyms <= hp.n; };
   auto load_chunk = [&](uint64_t c, uint4* d) {
-    const auto src = gmem(reinterpret_cast<const uint4*>(hp.text + c * kChunkSyms + 64ull * lane));
+    const u32x4_t* src = reinterpret_cast<const u32x4_t*>(hp.text + c * kChunkSyms + 64ull * lane);
 #pragma unroll
-    for (int i = 0; i < 4; ++i) d[i] = src[i];
+    for (int i = 0; i < 4; ++i) {
+      const u32x4_t v = ld_stream(src + i);
+      d[i] = make_uint4(v.x, v.y, v.z, v.w);
+    }
   };
   uint4 pf[4];
   uint64_t sc = blockIdx.x;
@@ -1479,8 +1481,15 @@ __global__ __launch_bounds__(64 * kSuper) void k_huff_super(HuffSuperParams hp, 
     for (int k = 0; k < G; ++k) P0[k] = 0;
     if (full_chunk(c)) {
       uint4 m[4];
+      if (hp.small_alpha) {
 #pragma unroll
-      for (int i = 0; i < 4; ++i) m[i] = make_uint4(map4(pf[i].x), map4(pf[i].y), map4(pf[i].z), map4(pf[i].w));
+        for (int i = 0; i < 4; ++i)
+          m[i] = make_uint4(__builtin_amdgcn_perm(thi, tlo, pf[i].x), __builtin_amdgcn_perm(thi, tlo, pf[i].y),
+                            __builtin_amdgcn_perm(thi, tlo, pf[i].z), __builtin_amdgcn_perm(thi, tlo, pf[i].w));
+      } else {
+#pragma unroll
+        for (int i = 0; i < 4; ++i) m[i] = make_uint4(map4(pf[i].x), map4(pf[i].y), map4(pf[i].z), map4(pf[i].w));
+      }
       uint64_t unused = 0;
       planes_from_bytes<G>(kp, m, P0, unused);
       if (full_chunk(cn)) load_chunk(cn, pf);  // prefetch
@@ -1538,10 +1547,13 @@ __global__ __launch_bounds__(64 * kSuper) void k_huff_super(HuffSuperParams hp, 
       stot[q] = __builtin_amdgcn_readlane(inc, WAVES - 1);
       spre[q] = __builtin_amdgcn_readlane(inc - v, uwid);
     }
+    uint32_t used[NSUB];  // the super-chunk's bits of each sub-level (its nodes)
+    uint32_t crow = 0;    // wave 0, lane s: the super-chunk's count of slot s
 #pragma unroll
     for (int j = 1; j <= JC; ++j) {
       const int nb = 1 << j;
       uint32_t T[1 << JC];
+      used[j - 1] = 0;
 #pragma unroll
       for (int q = 0; q < nb; ++q) {
         T[q] = 0;
@@ -1552,6 +1564,7 @@ __global__ __launch_bounds__(64 * kSuper) void k_huff_super(HuffSuperParams hp, 
       for (int q = 0; q < nb; ++q) {
         const uint32_t rk = hp.rank[j][q];
         if (rk == 0xffu) continue;  // no node
+        used[j - 1] += T[q];
         uint32_t start = 0, E = 0;
 #pragma unroll
         for (int q2 = 0; q2 < nb; ++q2) {
@@ -1564,16 +1577,18 @@ __global__ __launch_bounds__(64 * kSuper) void k_huff_super(HuffSuperParams hp, 
 #pragma unroll
         for (int i = 0; i < j; ++i) mq &= ((q >> i) & 1) ? P0[i] : ~P0[i];
         or_bits(img[j - 1], start + E, compress64h(P0[j], mq, lut), __popcll(mq));
-        if (uwid == 0 && lane == 0) gmem(hp.counts)[sc * hp.nslot + hp.slot0[j] + rk] = T[q];
+        crow = lane == int(hp.slot0[j] + rk) ? T[q] : crow;
       }
     }
     lds_sync();
 
-    const uint64_t s0 = sc * kSuperSyms;
-    const uint32_t nws = uint32_t((min<uint64_t>(kSuperSyms, hp.n - s0) + 63) >> 6);
+    // every slot is a node of one sub-level, each written above
+    if (uwid == 0 && uint32_t(lane) < hp.nslot) gmem(hp.counts)[sc * hp.nslot + lane] = crow;
+    // only the words that hold node bits are written (the stitch reads no
+    // bit outside a node's run)
 #pragma unroll
     for (int j = 0; j < NSUB; ++j) {
-      if (threadIdx.x < nws)
+      if (threadIdx.x < (used[j] + 63) >> 6)
         st_stream(hp.temp + j * hp.temp_stride + sc * kSuperWords + threadIdx.x, brev64(img[j][threadIdx.x]));
       img[j][threadIdx.x] = 0;
     }
