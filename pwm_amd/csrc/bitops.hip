@@ -83,10 +83,20 @@ __global__ __launch_bounds__(256) void k_leaf_hist(const T* text, uint64_t n, ui
     if (lh[i]) atomicAdd(&hist[i], (unsigned long long)lh[i]);
 }
 
-// One thread per destination word of a tree level; the word gathers the bits
-// of every bucket run that overlaps it (runs sorted by destination).
-__global__ void k_gather_runs(uint64_t* dst, const uint64_t* src, const uint64_t* dstart,
-                              const uint64_t* sstart, uint64_t nruns, uint64_t nbits) {
+// Every tree level l >= 1 in one launch (blockIdx.y = l - 1), one thread per
+// destination word; the word gathers the bits of every bucket run that
+// overlaps it (runs sorted by destination).  tab[3 l], tab[3 l + 1],
+// tab[3 l + 2]: where the level's run starts are in tab (destination starts,
+// nruns + 1 of them, then as many source starts), its number of runs, and
+// the matrix level's first word.
+__global__ void k_gather_runs(uint64_t* lv, uint64_t stride, const uint64_t* words, const uint64_t* tab,
+                              uint64_t nbits) {
+  const uint32_t l = blockIdx.y + 1;
+  const uint64_t nruns = tab[3 * l + 1];
+  const uint64_t* dstart = tab + tab[3 * l];
+  const uint64_t* sstart = dstart + nruns + 1;
+  const uint64_t* src = words + tab[3 * l + 2];
+  uint64_t* dst = lv + uint64_t(l - 1) * stride;
   const uint64_t w = uint64_t(blockIdx.x) * blockDim.x + threadIdx.x;
   const uint64_t nw = (nbits + 63) >> 6;
   if (w >= nw) return;
@@ -119,7 +129,7 @@ void wm_to_wt_device(pwmhip_ctx* ctx, const void* d_text, uint64_t n, uint32_t w
   const uint64_t K = 1ull << L;
   const uint64_t stride = level_stride(n);
   // scratch: leaf histogram, every level's run table, the reordered levels
-  const uint64_t tab = 2 * (K + 2 * uint64_t(L));
+  const uint64_t tab = 3 * uint64_t(L) + 2 * (K + 2 * uint64_t(L));
   ctx->scratch.reset();
   ctx->scratch.ensure((K + tab + uint64_t(L - 1) * stride) * 8 + 4096);
   unsigned long long* d_hist = ctx->scratch.take<unsigned long long>(K);
@@ -143,9 +153,10 @@ void wm_to_wt_device(pwmhip_ctx* ctx, const void* d_text, uint64_t n, uint32_t w
   // run tables of every level l >= 1, in destination (tree) order: runs of
   // the non-empty l-bit prefixes p, from matrix position (prefixes in
   // bit-reversed order) to tree position (numeric order)
-  std::vector<uint64_t> all;  // per level: ds[0..nruns], ss[0..nruns]
+  std::vector<uint64_t> all;  // a header of 3 words per level, then per level: ds[0..nruns], ss[0..nruns]
   std::vector<uint64_t> off(L), nr(L);
   all.reserve(tab);
+  all.assign(3 * uint64_t(L), 0);
   for (uint32_t l = L - 1; l >= 1; --l) {
     std::vector<uint64_t> up(1ull << l);
     for (uint64_t q = 0; q < up.size(); ++q) up[q] = pc[2 * q] + pc[2 * q + 1];
@@ -175,14 +186,15 @@ void wm_to_wt_device(pwmhip_ctx* ctx, const void* d_text, uint64_t n, uint32_t w
     all.insert(all.end(), ds.begin(), ds.end());
     all.insert(all.end(), ss.begin(), ss.end());
   }
+  for (uint32_t l = 1; l < L; ++l) {
+    all[3 * l] = off[l];
+    all[3 * l + 1] = nr[l];
+    all[3 * l + 2] = ws->level_offset[l];
+  }
   PWM_HIP_TRY(hipMemcpyAsync(d_tab, all.data(), all.size() * 8, hipMemcpyHostToDevice, st));
   const uint64_t nw = words_for(n);
-  for (uint32_t l = 1; l < L; ++l) {
-    const uint64_t* ds = d_tab + off[l];
-    hipLaunchKernelGGL(k_gather_runs, dim3(uint32_t((nw + 255) / 256)), dim3(256), 0, st,
-                       d_lv + uint64_t(l - 1) * stride, ws->d_words + ws->level_offset[l], ds, ds + nr[l] + 1,
-                       nr[l], n);
-  }
+  hipLaunchKernelGGL(k_gather_runs, dim3(uint32_t((nw + 255) / 256), L - 1), dim3(256), 0, st, d_lv, stride,
+                     ws->d_words, d_tab, n);
   PWM_HIP_TRY(hipGetLastError());
   // the tree levels replace the matrix levels in place (one copy when
   // levels 1.. are contiguous at one stride)
