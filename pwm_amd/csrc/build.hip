@@ -2366,6 +2366,25 @@ __global__ __launch_bounds__(256) void k_gather(GatherParams p) {
                W0, p.nwin, stage[threadIdx.x >> 6]);
 }
 
+// Sub-levels 1..g-1 of one pass in one launch (blockIdx.y = j - 1): level
+// j's entries at ((1 << j) - 2) * nchunks of the tables, its unit-local bits
+// at src + (j - 1) * src_stride, one plane each.
+struct GatherLevels {
+  GatherParams gp;   // j, dst and nplanes unused
+  uint64_t nchunks;
+  uint64_t* dst[kMaxGroup];
+};
+
+__global__ __launch_bounds__(256) void k_gather_levels(GatherLevels q) {
+  __shared__ unsigned long long stage[4][64];
+  const GatherParams& p = q.gp;
+  const uint32_t j = blockIdx.y + 1;
+  const uint64_t toff = uint64_t((1u << j) - 2) * q.nchunks;
+  const uint64_t W0 = (uint64_t(blockIdx.x) * 4 + (threadIdx.x >> 6)) * 64 * p.nwin;
+  gather_range(p.tab_dst + toff, p.tab_src + toff, p.level_ne[j], p.src + (j - 1) * p.src_stride, 0, q.dst[j], 0,
+               1, p.n, W0, p.nwin, stage[threadIdx.x >> 6]);
+}
+
 // Batch: waves are assigned to texts by wave_base (per text), level j.
 struct BatchGatherParams {
   const uint64_t* wave_base;   // ntexts + 1
@@ -3264,19 +3283,18 @@ void build_passes_device(pwmhip_ctx* ctx, const void* d_text, uint64_t n, uint32
       const uint64_t W = words_for(n);
       const uint32_t nwin = gather_windows(W, ctx->num_cus);
       const uint32_t gblocks = uint32_t((W + 256ull * nwin - 1) / (256ull * nwin));
-      for (uint32_t j = 1; !sup && j < g; ++j) {
-        GatherParams gp{};
-        gp.nwin = nwin;
-        const uint64_t toff = uint64_t((1u << j) - 2) * nchunks;
-        gp.tab_dst = U64(9) + toff;
-        gp.tab_src = U64(10) + toff;
-        gp.level_ne = U64(8);
-        gp.j = j;
-        gp.n = n;
-        gp.src = U64(0) + (j - 1) * stride;
-        gp.dst = ws->d_words + ws->level_offset[l0 + j];
-        gp.nplanes = 1;
-        hipLaunchKernelGGL(k_gather, dim3(gblocks), dim3(256), 0, s, gp);
+      if (!sup && g > 1) {
+        GatherLevels gl{};
+        gl.gp.nwin = nwin;
+        gl.gp.tab_dst = U64(9);
+        gl.gp.tab_src = U64(10);
+        gl.gp.level_ne = U64(8);
+        gl.gp.n = n;
+        gl.gp.src = U64(0);
+        gl.gp.src_stride = stride;
+        gl.nchunks = nchunks;
+        for (uint32_t j = 1; j < g; ++j) gl.dst[j] = ws->d_words + ws->level_offset[l0 + j];
+        hipLaunchKernelGGL(k_gather_levels, dim3(gblocks, g - 1), dim3(256), 0, s, gl);
       }
       if (!last) {
         ScatterParams sp{};
