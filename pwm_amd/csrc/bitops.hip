@@ -130,11 +130,20 @@ void wm_to_wt_device(pwmhip_ctx* ctx, const void* d_text, uint64_t n, uint32_t w
   const uint64_t stride = level_stride(n);
   // scratch: leaf histogram, every level's run table, the reordered levels
   const uint64_t tab = 3 * uint64_t(L) + 2 * (K + 2 * uint64_t(L));
+  // levels 1.. of the matrix (the sources) and of the output contiguous at
+  // one stride, and at most kMaxGroup - 1 of them: the plain builds' wide
+  // stitch moves the runs (tables in its layout: level l's entries from
+  // 2^l - 2); else one thread per destination word (k_gather_runs)
+  bool contiguous = true;
+  for (uint32_t l = 2; l < L; ++l) contiguous &= ws->level_offset[l] == ws->level_offset[1] + (l - 1) * stride;
+  const bool wide = contiguous && L - 1 < uint32_t(kMaxGroup);
+  const uint64_t fw = wide ? wide_stitch_first_words(n, L - 1) : 0;
   ctx->scratch.reset();
-  ctx->scratch.ensure((K + tab + uint64_t(L - 1) * stride) * 8 + 4096);
+  ctx->scratch.ensure((K + tab + uint64_t(L - 1) * stride) * 8 + fw * 4 + 4096);
   unsigned long long* d_hist = ctx->scratch.take<unsigned long long>(K);
   uint64_t* d_tab = ctx->scratch.take<uint64_t>(tab);
   uint64_t* d_lv = ctx->scratch.take<uint64_t>(uint64_t(L - 1) * stride);
+  uint32_t* d_first = wide ? ctx->scratch.take<uint32_t>(fw) : nullptr;
   PWM_HIP_TRY(hipMemsetAsync(d_hist, 0, K * 8, st));
   // (a grid-stride grid that is resident at once: the kernel's registers
   // leave room for five waves a SIMD)
@@ -157,6 +166,11 @@ void wm_to_wt_device(pwmhip_ctx* ctx, const void* d_text, uint64_t n, uint32_t w
   std::vector<uint64_t> off(L), nr(L);
   all.reserve(tab);
   all.assign(3 * uint64_t(L), 0);
+  // the wide stitch's layout: entries per level, destination starts, source starts
+  std::vector<uint64_t> wtab(wide ? kMaxGroup + 2 * K : 0, 0);
+  uint64_t* w_ne = wtab.data();
+  uint64_t* w_dst = w_ne + kMaxGroup;
+  uint64_t* w_src = w_dst + K;
   for (uint32_t l = L - 1; l >= 1; --l) {
     std::vector<uint64_t> up(1ull << l);
     for (uint64_t q = 0; q < up.size(); ++q) up[q] = pc[2 * q] + pc[2 * q + 1];
@@ -183,23 +197,47 @@ void wm_to_wt_device(pwmhip_ctx* ctx, const void* d_text, uint64_t n, uint32_t w
     ss.push_back(0);
     off[l] = all.size();
     nr[l] = ds.size() - 1;
+    if (wide) {
+      w_ne[l] = nr[l];
+      for (uint64_t r = 0; r < nr[l]; ++r) {
+        w_dst[(1ull << l) - 2 + r] = ds[r];
+        w_src[(1ull << l) - 2 + r] = ss[r];
+      }
+      continue;
+    }
     all.insert(all.end(), ds.begin(), ds.end());
     all.insert(all.end(), ss.begin(), ss.end());
   }
-  for (uint32_t l = 1; l < L; ++l) {
-    all[3 * l] = off[l];
-    all[3 * l + 1] = nr[l];
-    all[3 * l + 2] = ws->level_offset[l];
-  }
-  PWM_HIP_TRY(hipMemcpyAsync(d_tab, all.data(), all.size() * 8, hipMemcpyHostToDevice, st));
   const uint64_t nw = words_for(n);
-  hipLaunchKernelGGL(k_gather_runs, dim3(uint32_t((nw + 255) / 256), L - 1), dim3(256), 0, st, d_lv, stride,
-                     ws->d_words, d_tab, n);
-  PWM_HIP_TRY(hipGetLastError());
+  if (wide) {
+    PWM_HIP_TRY(hipMemcpyAsync(d_tab, wtab.data(), wtab.size() * 8, hipMemcpyHostToDevice, st));
+    WideStitch w{};
+    w.level_ne = d_tab;
+    w.tab_dst = d_tab + kMaxGroup;
+    w.tab_src = d_tab + kMaxGroup + K;
+    w.first = d_first;
+    w.src = ws->d_words + ws->level_offset[1];
+    w.src_stride = stride;
+    for (uint32_t j = 1; j < L; ++j) {
+      w.dst[j] = d_lv + uint64_t(j - 1) * stride;
+      w.nbits[j] = n;
+    }
+    w.units = 1;
+    w.nsub = L - 1;
+    launch_wide_stitch(ctx, w);
+  } else {
+    for (uint32_t l = 1; l < L; ++l) {
+      all[3 * l] = off[l];
+      all[3 * l + 1] = nr[l];
+      all[3 * l + 2] = ws->level_offset[l];
+    }
+    PWM_HIP_TRY(hipMemcpyAsync(d_tab, all.data(), all.size() * 8, hipMemcpyHostToDevice, st));
+    hipLaunchKernelGGL(k_gather_runs, dim3(uint32_t((nw + 255) / 256), L - 1), dim3(256), 0, st, d_lv, stride,
+                       ws->d_words, d_tab, n);
+    PWM_HIP_TRY(hipGetLastError());
+  }
   // the tree levels replace the matrix levels in place (one copy when
   // levels 1.. are contiguous at one stride)
-  bool contiguous = true;
-  for (uint32_t l = 2; l < L; ++l) contiguous &= ws->level_offset[l] == ws->level_offset[1] + (l - 1) * stride;
   if (contiguous) {
     PWM_HIP_TRY(hipMemcpyAsync(ws->d_words + ws->level_offset[1], d_lv, uint64_t(L - 1) * stride * 8,
                                hipMemcpyDeviceToDevice, st));
