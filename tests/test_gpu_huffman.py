@@ -130,6 +130,31 @@ def test_huffman_histogram_small_alphabet_with_rare_outliers():
 
 
 @pytest.mark.gpu
+def test_huffman_histogram_packed_counts_large():
+    """Texts past one grid step of the small-alphabet histogram (16 MiB and
+    more) are counted 16 bytes at a time: all below 4 (2-bit fields), some
+    value 4-7 (3-bit fields) or a byte of 8 or more (byte by byte), the three
+    kinds interleaved at 16-byte granularity; per-level sizes and zeros
+    follow from the counts, so a miscount shows."""
+    rng = np.random.default_rng(0xB18)
+    n = (40 << 20) + 4321
+    t = rng.integers(0, 4, n, dtype=np.uint8)
+    for b in rng.choice(n // 65536, 40, replace=False):  # 3-bit stretches
+        t[b * 65536:(b + 1) * 65536] = rng.integers(0, 8, 65536, dtype=np.uint8)
+    g = rng.choice(n // 16, 5000, replace=False)  # lone 16-byte groups with a 4-7
+    t[g * 16 + rng.integers(0, 16, g.size)] = rng.integers(4, 8, g.size, dtype=np.uint8)
+    pos = (1 << 20) + rng.choice(n - (1 << 20), 300, replace=False)  # past the sample
+    t[pos] = rng.integers(8, 256, pos.size, dtype=np.uint8)
+    for off in (0, 5):
+        v = t[off:]
+        o = O.huff_naive(v, False)
+        ws = pwm_amd.build(v, 0, "huff_naive", False)
+        _same(ws, o.levels, o.bits, ("packed", off))
+        assert np.array_equal(ws.zeros(), o.zeros), off
+        ws.free()
+
+
+@pytest.mark.gpu
 def test_huffman_reused_output():
     """Huffman builds into an earlier structure's memory (large enough: taken
     over; too small: replaced), results unchanged."""
