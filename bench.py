@@ -12,9 +12,15 @@ step); `roofline` is that of the dominant kernel, the local build (it reads
 the text once and writes every level's bits once: n*width + levels*n/8
 bytes), over its own device time (HIP events on the build stream);
 `config.build_roofline` relates the same algorithmic bytes to the device
-time of the whole build; `cpu_baseline` is the reference's own OpenMP algorithms
-(compiled from its headers, oracle/_ref) on a bounded prefix of the same
-text, on this host's cores.
+time of the whole build; `verified` says whether the last build equals the
+reference's own output on the same text (per-level SHA-256 digests, zeros,
+codes; tests/golden/fullsize_digests.json), checked after the timed region;
+on the headline workload `batched` holds the many-texts comparison (20,000
+uneven DNA texts: one batched build against one plain build per text, the
+checked structures against per-text builds and the oracle); `cpu_baseline`
+is the reference's own OpenMP algorithms (compiled from its headers,
+oracle/_ref) on a bounded prefix of the same text, on this host's cores.
+The process exits with status 1 if a verification fails.
 """
 import argparse
 import json
@@ -43,6 +49,8 @@ CONFIGS = {
 STREAM_CONFIGS = {"dna4g_stream": "dna4g_wm"}
 # random symbol queries timed after the build
 QUERIES = {"zipf1g_wm": 10**7}
+# workloads whose line also carries the many-texts figures
+BATCH_WITH = {"dna4g_wm": "dna_batch20k"}
 
 
 # Many uneven texts, built together: (seed, texts, median length, sigma,
@@ -52,33 +60,51 @@ BATCH_CONFIGS = {
 }
 
 
+def _omp_env(cores):
+    """OpenMP placement for the reference's CPU algorithms: one thread per
+    core, spread over the sockets (read by libgomp when the compiled
+    reference is first loaded)."""
+    os.environ.setdefault("OMP_NUM_THREADS", str(cores))
+    os.environ.setdefault("OMP_PROC_BIND", "spread")
+    os.environ.setdefault("OMP_PLACES", "cores")
+
+
+def _timed_runs(fn, runs=5):
+    """One warm-up call, then `runs` timed calls: (true median, minimum)."""
+    fn()
+    ts = sorted(fn() for _ in range(runs))
+    return ts[len(ts) // 2], ts[0]
+
+
 def cpu_baseline(text, levels, tree, width, sample_bytes, algo="ppc"):
     """Reference algorithms (wx_ppc / wx_pps / wx_dd_pc, or wx_huff_pc for a
-    Huffman-shaped build) on a prefix sample."""
+    Huffman-shaped build) on a prefix sample: one warm-up and five timed
+    runs each, true median (the reference's own "median" is the middle run
+    by start time, stats.hpp:162-164)."""
     sys.path.insert(0, os.path.join(ROOT, "tests"))
-    import oracle_lib
     cores = len(os.sched_getaffinity(0))
+    _omp_env(cores)
+    import oracle_lib
     sample = np.ascontiguousarray(text[: sample_bytes // width])
     if oracle_lib.have_ref() and algo.startswith("huff"):
-        oracle_lib.ref_build("huff_pc", sample[: 1 << 20], 0, tree)  # warm-up
-        ts = sorted(oracle_lib.ref_build("huff_pc", sample, 0, tree)[1] for _ in range(3))
-        return {"value": round(sample.nbytes / ts[1] / 1e9, 3), "unit": "GB/s", "cores": 1, "kind": "reference",
+        med, mn = _timed_runs(lambda: oracle_lib.ref_build("huff_pc", sample, 0, tree)[1])
+        return {"value": round(sample.nbytes / med / 1e9, 3), "unit": "GB/s", "cores": 1, "kind": "reference",
                 "sample": "%d MiB prefix of the same text; wx_huff_pc (sequential in the reference), "
-                          "median of 3" % (sample.nbytes >> 20)}
+                          "1 warm-up + 5 runs, median (min %.3f s)" % (sample.nbytes >> 20, mn)}
     if oracle_lib.have_ref():
-        os.environ.setdefault("OMP_NUM_THREADS", str(cores))
         best = None
         per = {}
-        for algo in ("ppc", "pps", "dd_pc"):
-            oracle_lib.ref_build(algo, sample, levels, tree)  # warm-up
-            ts = sorted(oracle_lib.ref_build(algo, sample, levels, tree)[1] for _ in range(3))
-            per[algo] = round(sample.nbytes / ts[1] / 1e9, 3)
-            if best is None or per[algo] > best[1]:
-                best = (algo, per[algo])
+        for a in ("ppc", "pps", "dd_pc"):
+            med, mn = _timed_runs(lambda: oracle_lib.ref_build(a, sample, levels, tree)[1])
+            per[a] = {"median_GBps": round(sample.nbytes / med / 1e9, 3), "best_GBps": round(sample.nbytes / mn / 1e9, 3)}
+            if best is None or per[a]["median_GBps"] > best[1]:
+                best = (a, per[a]["median_GBps"])
         return {"value": best[1], "unit": "GB/s", "cores": int(os.environ["OMP_NUM_THREADS"]),
                 "kind": "reference",
-                "sample": "%d MiB prefix of the same text; wx_%s best of ppc/pps/dd_pc (median of 3): %s"
-                          % (sample.nbytes >> 20, best[0], per)}
+                "sample": "%d MiB prefix of the same text; wx_%s, best of ppc/pps/dd_pc by median of 5 runs "
+                          "after 1 warm-up, OMP_PROC_BIND=%s OMP_PLACES=%s: %s"
+                          % (sample.nbytes >> 20, best[0], os.environ["OMP_PROC_BIND"], os.environ["OMP_PLACES"],
+                             per)}
     small = sample[: 16 << 20]
     t0 = time.perf_counter()
     if algo.startswith("huff"):
@@ -99,16 +125,79 @@ def traffic_from_profiles(config):
     return None
 
 
-def rooflines(b_min, phases_ms, total_ms, config):
-    """Dominant kernel (local build, phase 0) and whole-build figures."""
-    k1 = b_min / (phases_ms[0] / 1e3) / 1e9
+def rooflines(b_min, phases_ms, total_ms, config, passes=1):
+    """Dominant kernel (local build, phase 0) and whole-build figures.  With
+    one pass the local build reads the text once and writes every level
+    once, so its algorithmic bytes are the build's B_min.  A multi-pass
+    build's phase-0 time is its first pass only and does a fraction of B_min;
+    there the whole build is reported against B_min instead."""
     build = b_min / (total_ms / 1e3) / 1e9
-    roof = {"bound": "hbm", "kernel": "local build (k_local_super / k_local_small)", "achieved": round(k1, 1),
-            "peak": HBM_PEAK_GBS, "unit": "GB/s", "frac": round(k1 / HBM_PEAK_GBS, 4),
-            "traffic": traffic_from_profiles(config), "algorithmic_bytes": int(b_min),
-            "kernel_ms": round(phases_ms[0], 4)}
     whole = {"achieved": round(build, 1), "frac": round(build / HBM_PEAK_GBS, 4), "device_ms": round(total_ms, 4)}
+    if passes == 1:
+        k1 = b_min / (phases_ms[0] / 1e3) / 1e9
+        roof = {"bound": "hbm", "kernel": "local build (k_local_super / k_local_small)", "achieved": round(k1, 1),
+                "peak": HBM_PEAK_GBS, "unit": "GB/s", "frac": round(k1 / HBM_PEAK_GBS, 4),
+                "traffic": traffic_from_profiles(config), "algorithmic_bytes": int(b_min),
+                "kernel_ms": round(phases_ms[0], 4)}
+    else:
+        roof = {"bound": "hbm", "kernel": "whole build (%d passes; no single launch does the build's bytes)" % passes,
+                "achieved": whole["achieved"], "peak": HBM_PEAK_GBS, "unit": "GB/s", "frac": whole["frac"],
+                "traffic": None, "algorithmic_bytes": int(b_min), "kernel_ms": round(total_ms, 4)}
     return roof, whole
+
+
+# Full-size digests of the reference's own outputs (tests/golden/
+# fullsize_digests.json): workload -> (text entry, build entry)
+DIGESTS = {
+    "dna4g_wm": ("dna4g", "wm_ppc"), "dna4g_wt": ("dna4g", "wt_ppc"), "dna4g_huffwm": ("dna4g", "wm_huff_pc"),
+    "zipf1g_wm": ("zipf256m", "wm_pps"), "uniform64m_wm": ("uniform64m", "wm_ppc"),
+    "uniform64m_wt": ("uniform64m", "wt_pps"), "dna4g_stream": ("dna4g", "wm_ppc"),
+}
+
+
+def verify_against_reference(config, text, ws, lib, ctx):
+    """Compares the last build with the reference's own output on the same
+    text: per-level SHA-256 of the words, level sizes, zeros and (Huffman)
+    codes.  Runs outside the timed region."""
+    import ctypes
+    import hashlib
+    if config not in DIGESTS:
+        return {"verified": None, "why": "no reference digest for this workload"}
+    with open(os.path.join(ROOT, "tests", "golden", "fullsize_digests.json")) as f:
+        dig = json.load(f)
+    tname, bname = DIGESTS[config]
+    ent = dig[tname]
+    if text.size != ent["symbols"]:
+        return {"verified": None, "why": "text length differs from the digested workload"}
+
+    def sha(a):
+        return hashlib.sha256(memoryview(np.ascontiguousarray(a)).cast("B")).hexdigest()
+
+    rec = ent["builds"][bname]
+    bad = []
+    if sha(text) != ent["text_sha256"]:
+        bad.append("text")
+    L = ws.levels()
+    if L != len(rec["level_bits"]) or [ws.level_bit_size(l) for l in range(L)] != rec["level_bits"]:
+        bad.append("level sizes")
+    else:
+        for l in range(L):
+            a = np.empty((ws.level_bit_size(l) + 63) // 64, np.uint64)
+            st = lib.pwmhip_ws_download(ctx.h, ctypes.byref(ws._ws), l, a.ctypes.data_as(ctypes.c_void_p))
+            if st or sha(a) != rec["level_sha256"][l]:
+                bad.append("level %d" % l)
+    if [int(z) for z in ws.zeros()] != rec["zeros"]:
+        bad.append("zeros")
+    if "code_len" in rec:
+        ln, wd = ws.codes()
+        if [int(x) for x in ln] != rec["code_len"] or [int(x) for x in wd] != rec["code_word"]:
+            bad.append("codes")
+    out = {"verified": not bad,
+           "against": "reference wx_%s on the same text: per-level SHA-256, level sizes, zeros%s "
+                      "(tests/golden/fullsize_digests.json)" % (bname[3:], ", codes" if "code_len" in rec else "")}
+    if bad:
+        out["mismatch"] = bad
+    return out
 
 
 def batch_sizes(seed, T, median, sigma, lo, hi):
@@ -124,7 +213,7 @@ def cpu_baseline_batch(host, offs, sizes, levels, tree, sample_bytes):
     if not oracle_lib.have_ref():
         return None
     cores = len(os.sched_getaffinity(0))
-    os.environ.setdefault("OMP_NUM_THREADS", str(cores))
+    _omp_env(cores)
     k = int(np.searchsorted(np.cumsum(sizes), sample_bytes)) + 1
     k = min(k, len(sizes))
     sample = [host[offs[i]:offs[i] + sizes[i]] for i in range(k)]
@@ -141,13 +230,14 @@ def cpu_baseline_batch(host, offs, sizes, levels, tree, sample_bytes):
                       "ppc/pps/dd_pc: %s" % (k, nbytes >> 20, best[0], per)}
 
 
-def run_batch(args):
-    """Batched construction of many uneven texts against one build per text."""
+def batch_figures(config, steps, warmup, graph=True, per_text=True, check_oracle=True):
+    """Batched construction of many uneven texts against one build per text:
+    returns (figures dict, b_min, phases, device ms, host texts)."""
     import ctypes
     import torch
     import pwm_amd
 
-    seed, T, median, sigma, lo, hi, levels, algo, tree = BATCH_CONFIGS[args.config]
+    seed, T, median, sigma, lo, hi, levels, algo, tree = BATCH_CONFIGS[config]
     sizes = batch_sizes(seed, T, median, sigma, lo, hi)
     padded = (sizes + 255) // 256 * 256
     offs = np.concatenate([[0], np.cumsum(padded)[:-1]]).astype(np.int64)
@@ -176,16 +266,16 @@ def run_batch(args):
         dev = []
         torch.cuda.synchronize()
         t0 = time.perf_counter()
-        for _ in range(args.steps):
+        for _ in range(steps):
             batched(extra)
             dev.append(ctx.last_timing())
         torch.cuda.synchronize()
-        return (time.perf_counter() - t0) / args.steps * 1e3, dev
+        return (time.perf_counter() - t0) / steps * 1e3, dev
 
     # the launch variants are tuned on the first call; the captured graph is
     # replayed from the second on
-    graph_flags = 0 if args.no_graph else pwm_amd.FLAG_GRAPH | pwm_amd.FLAG_AUTOTUNE
-    for _ in range(max(args.warmup, 1)):
+    graph_flags = pwm_amd.FLAG_GRAPH | pwm_amd.FLAG_AUTOTUNE if graph else 0
+    for _ in range(max(warmup, 1)):
         batched(graph_flags)
     batch_ms, dev = timed(graph_flags)
     dev_ms = float(np.mean([d[3] for d in dev]))
@@ -193,15 +283,16 @@ def run_batch(args):
     # timed as a whole)
     eager_ms, edev = timed(0)
     phases = [round(float(np.mean([d[i] for d in edev])), 4) for i in range(3)]
+    batched(graph_flags)  # the checked structures come from a graph replay
 
     # one plain build per text, each into its own reused output
     singles = (pwm_amd._WS * T)()
     sflags = flags | pwm_amd.FLAG_REUSE_OUTPUT
 
-    def per_text():
+    def per_text_builds(idx):
         devsum = 0.0
         arr = (ctypes.c_double * 4)()
-        for i in range(T):
+        for i in idx:
             st = lib.pwmhip_build(ctx.h, ptrs[i], csz[i], 1, levels, code, sflags, ctypes.byref(singles[i]))
             if st != pwm_amd.OK:
                 raise pwm_amd.PwmError(st, "build")
@@ -212,75 +303,105 @@ def run_batch(args):
     rng = np.random.default_rng(1)
     check = sorted(set(rng.integers(0, T, 200).tolist()) | {int(np.argmax(sizes)), int(np.argmin(sizes))})
     single_ms = single_dev_ms = None
-    if not args.no_per_text:
-        per_text()  # allocates every output once
+    if per_text:
+        per_text_builds(range(T))  # allocates every output once
         torch.cuda.synchronize()
         t0 = time.perf_counter()
-        single_dev_ms = per_text()
+        single_dev_ms = per_text_builds(range(T))
         torch.cuda.synchronize()
         single_ms = (time.perf_counter() - t0) * 1e3
     else:
-        # only the checked texts, for the comparison below
-        for i in check:
-            st = lib.pwmhip_build(ctx.h, ptrs[i], csz[i], 1, levels, code, sflags, ctypes.byref(singles[i]))
-            if st != pwm_amd.OK:
-                raise pwm_amd.PwmError(st, "build")
+        per_text_builds(check)
 
-    # bit-exact: the batched structures equal the per-text ones
+    # bit-exact: the batched structures equal the per-text ones and the
+    # oracle's (wx_naive restated; zeros[levels-1] is 0 for ppc)
+    if check_oracle:
+        sys.path.insert(0, os.path.join(ROOT, "tests"))
+        import oracle_lib
+    mism = []
     for i in check:
         a, b = outs[i], singles[i]
-        assert a.levels == b.levels and list(a.zeros[:levels]) == list(b.zeros[:levels]), i
-        for l in range(a.levels):
+        ok = a.levels == b.levels and list(a.zeros[:levels]) == list(b.zeros[:levels])
+        t_i = host[offs[i]:offs[i] + sizes[i]]
+        o = oracle_lib.naive(t_i, levels, tree) if check_oracle else None
+        if o is not None and not tree:
+            z = o.zeros.copy()
+            if algo in ("ppc", "ppc_ss", "pc_ss", "dd_pc_ss"):
+                z[-1] = 0
+            ok = ok and list(a.zeros[:levels]) == [int(x) for x in z]
+        for l in range(a.levels if ok else 0):
             nw = (int(a.level_bits[l]) + 63) // 64
             x = np.empty(nw, np.uint64)
             y = np.empty(nw, np.uint64)
             lib.pwmhip_ws_download(ctx.h, ctypes.byref(a), l, x.ctypes.data_as(ctypes.c_void_p))
             lib.pwmhip_ws_download(ctx.h, ctypes.byref(b), l, y.ctypes.data_as(ctypes.c_void_p))
-            assert np.array_equal(x, y), (i, l)
+            ok = ok and np.array_equal(x, y) and (o is None or np.array_equal(x, o.levels[l]))
+        if not ok:
+            mism.append(i)
 
     words = ((sizes + 63) // 64).sum()
     b_min = total + levels * 8 * int(words)
-    roof, whole = rooflines(b_min, phases, dev_ms, args.config)
-    out = {
-        "metric": "WM/WT build input GB/s + % HBM peak; batched vs per-text launches, bit-exact",
-        "value": round(total / (batch_ms / 1e3) / 1e9, 2),
-        "unit": "GB/s",
-        "n_gpus": 1,
-        "steps": args.steps,
-        "warmup": args.warmup,
-        "ms_per_step": round(batch_ms, 4),
-        "higher_is_better": True,
-        "scaling": "weak",
-        "vs_baseline": None,
-        "dtype": "u8",
-        "data": "synthetic (dna generator, seed 0x%X; lognormal lengths)" % seed,
-        "config": {"workload": "%s: %d uint8 DNA texts, lengths lognormal(median %d, sigma %.1f) clipped "
-                               "[%d, %d], %d symbols total, %d levels, %s %s, one batched build vs one "
-                               "build per text" % (args.config, T, median, sigma, lo, hi, total, levels,
-                                                   "wt" if tree else "wm", algo),
-                   "texts": T, "symbols": total, "levels": levels,
-                   "algorithm": ("wt_" if tree else "wm_") + algo,
-                   "device_ms_per_build": round(dev_ms, 4),
-                   "graph": not args.no_graph,
-                   "eager_ms_per_build": round(eager_ms, 4),
-                   "eager_device_ms": round(float(np.mean([d[3] for d in edev])), 4),
-                   "device_ms_phases": {"local_build": phases[0], "offsets": phases[1], "stitch": phases[2]},
-                   "per_text_ms": round(single_ms, 2) if single_ms else None,
-                   "per_text_device_ms": round(single_dev_ms, 2) if single_dev_ms else None,
-                   "speedup_vs_per_text": round(single_ms / batch_ms, 2) if single_ms else None,
-                   "checked_texts": len(check),
-                   "build_roofline": whole,
-                   "generate_s": round(gen_s, 2)},
-        "roofline": roof,
+    figures = {
+        "texts": T, "symbols": total, "levels": levels, "algorithm": ("wt_" if tree else "wm_") + algo,
+        "batched_ms": round(batch_ms, 4),
+        "batched_device_ms": round(dev_ms, 4),
+        "graph": graph,
+        "eager_ms": round(eager_ms, 4),
+        "device_ms_phases": {"local_build": phases[0], "offsets": phases[1], "stitch": phases[2]},
+        "batched_input_GBps": round(total / (batch_ms / 1e3) / 1e9, 2),
+        "per_text_ms": round(single_ms, 2) if single_ms else None,
+        "per_text_device_ms": round(single_dev_ms, 2) if single_dev_ms else None,
+        "speedup_vs_per_text": round(single_ms / batch_ms, 2) if single_ms else None,
+        "checked_texts": len(check),
+        "verified": not mism,
+        "checked_against": "per-text builds%s, every level word and zeros" % (" and the oracle" if check_oracle else ""),
+        "generate_s": round(gen_s, 2),
     }
-    if not args.no_cpu:
-        out["cpu_baseline"] = cpu_baseline_batch(host, offs, sizes, levels, tree, args.cpu_sample_mib << 20)
+    if mism:
+        figures["mismatch_texts"] = mism[:20]
     for i in range(T):
         lib.pwmhip_ws_free(ctx.h, ctypes.byref(singles[i]))
     for i in range(T - 1, -1, -1):
         lib.pwmhip_ws_free(ctx.h, ctypes.byref(outs[i]))
     ctx.close()
+    del d_all
+    torch.cuda.empty_cache()
+    return figures, b_min, phases, dev_ms, (host, offs, sizes)
+
+
+def run_batch(args):
+    """The many-texts workload as its own line."""
+    seed, T, median, sigma, lo, hi, levels, algo, tree = BATCH_CONFIGS[args.config]
+    fig, b_min, phases, dev_ms, (host, offs, sizes) = batch_figures(
+        args.config, args.steps, args.warmup, graph=not args.no_graph, per_text=not args.no_per_text)
+    roof, whole = rooflines(b_min, phases, dev_ms, args.config)
+    fig["build_roofline"] = whole
+    fig["workload"] = ("%s: %d uint8 DNA texts, lengths lognormal(median %d, sigma %.1f) clipped [%d, %d], "
+                       "%d symbols total, %d levels, %s %s, one batched build vs one build per text"
+                       % (args.config, T, median, sigma, lo, hi, fig["symbols"], levels, "wt" if tree else "wm",
+                          algo))
+    out = {
+        "metric": "WM/WT build input GB/s + % HBM peak; batched vs per-text launches, bit-exact",
+        "value": fig["batched_input_GBps"],
+        "unit": "GB/s",
+        "n_gpus": 1,
+        "steps": args.steps,
+        "warmup": args.warmup,
+        "ms_per_step": fig["batched_ms"],
+        "higher_is_better": True,
+        "scaling": "weak",
+        "vs_baseline": None,
+        "dtype": "u8",
+        "data": "synthetic (dna generator, seed 0x%X; lognormal lengths)" % seed,
+        "verified": fig["verified"],
+        "config": fig,
+        "roofline": roof,
+    }
+    if not args.no_cpu:
+        out["cpu_baseline"] = cpu_baseline_batch(host, offs, sizes, levels, tree, args.cpu_sample_mib << 20)
     print(json.dumps(out), flush=True)
+    if not fig["verified"]:
+        sys.exit(1)
 
 
 # Serving: Poisson arrivals of build+query requests (seed, requests per rate,
@@ -417,7 +538,9 @@ def main():
     ap.add_argument("--queries", type=int, default=None, help="random symbol queries to time after the build")
     ap.add_argument("--no-per-text", action="store_true",
                     help="many-texts configs: skip the one-build-per-text comparison")
-    ap.add_argument("--cpu-sample-mib", type=int, default=256)
+    ap.add_argument("--cpu-sample-mib", type=int, default=1024)
+    ap.add_argument("--no-batch", action="store_true",
+                    help="headline workload: skip the many-texts (batched vs per-text) figures")
     ap.add_argument("--no-graph", action="store_true", help="batches: eager launches, default variants")
     ap.add_argument("--copy-mib", type=int, default=0,
                     help="only time a plain device-to-device copy of this many MiB (calibrates the "
@@ -483,17 +606,14 @@ def main():
     dev_total = float(np.mean([d[3] for d in dev_ms]))
     phases = [round(float(np.mean([d[i] for d in dev_ms])), 4) for i in range(3)]
 
-    # light self-check of the last build: zeros against a host histogram
-    z = ws.zeros()
-    if not tree and levels:
-        hist = np.bincount(text[: min(n, 1 << 28)], minlength=1 << levels) if width == 1 and n <= (1 << 28) else None
-        if hist is not None:
-            exp0 = int(hist[: 1 << (levels - 1)].sum())
-            assert int(z[0]) == exp0, "zeros[0] mismatch"
+    # the last build against the reference's own output on the same text
+    # (outside the timed region)
+    ver = verify_against_reference(args.config, text, ws, pwm_amd.lib(), ctx)
 
     # text read once + every level's words written once (Huffman levels are ragged)
     b_min = n * width + 8 * sum((ws.level_bit_size(l) + 63) // 64 for l in range(ws.levels()))
-    roof, whole = rooflines(b_min, phases, dev_total, args.config)
+    passes = 1 if (levels <= 4 or algo.startswith("huff")) else (levels + 3) // 4
+    roof, whole = rooflines(b_min, phases, dev_total, args.config, passes)
     value = n * width / (ms_per_step / 1e3) / 1e9
     out = {
         "metric": "WM/WT build input GB/s + % HBM peak; batched vs per-text launches, bit-exact",
@@ -508,6 +628,7 @@ def main():
         "vs_baseline": None,
         "dtype": "u%d" % (8 * width),
         "data": "synthetic (%s generator, seed 0x%X)" % (kind, seed),
+        "verified": ver["verified"],
         "config": {"workload": "%s: %d x uint%d, %d levels, %s %s" % (
             args.config, n, 8 * width, ws.levels(), "wt" if tree else "wm", algo),
             "symbols": n, "levels": ws.levels(), "algorithm": ("wt_" if tree else "wm_") + algo,
@@ -517,17 +638,30 @@ def main():
             "text": "pinned host memory, streamed in 256 Mi-symbol slices" if streamed else "resident in HBM",
             "plain_h2d_copy_GBps": h2d_gbps,
             "build_roofline": whole,
+            "verification": ver,
             "generate_s": round(gen_s, 2)},
         "roofline": roof,
     }
     nq = args.queries if args.queries is not None else QUERIES.get(args.config, 0)
     if nq:
         out["config"]["queries"] = time_queries(pwm_amd, ctx, ws, text, nq)
-    if not args.no_cpu:
-        out["cpu_baseline"] = cpu_baseline(text, levels, tree, width, args.cpu_sample_mib << 20, algo)
     ws.free()
     ctx.close()
+    del src
+    torch.cuda.empty_cache()
+    # the headline workload's line also carries the many-texts comparison
+    # (batched launches against one plain launch per text, same exactness)
+    if args.config in BATCH_WITH and not args.no_batch:
+        bname = BATCH_WITH[args.config]
+        fig = batch_figures(bname, max(3, args.steps // 2), max(1, args.warmup // 2))[0]
+        fig["workload"] = bname
+        out["batched"] = fig
+        out["verified"] = bool(out["verified"]) and fig["verified"]
+    if not args.no_cpu:
+        out["cpu_baseline"] = cpu_baseline(text, levels, tree, width, args.cpu_sample_mib << 20, algo)
     print(json.dumps(out), flush=True)
+    if out["verified"] is False:
+        sys.exit(1)
 
 
 def time_queries(pwm_amd, ctx, ws, text, nq, checked=20000):

@@ -135,8 +135,10 @@ int pwmhip_ws_download(pwmhip_ctx* ctx, const pwmhip_ws* ws, uint64_t level, uin
  * bin_select_popcnt<0/1> (bin_select_popcnt.hpp:38-107) directories.
  * Tables: per level, 64 Ki-bit superblocks with absolute uint64 counts and
  * 512-bit blocks with uint16 counts relative to their superblock, plus
- * select samples every 8192 ones and zeros.  Valid for levels of any size
- * below 2^63 bits. */
+ * select samples every 8192 ones and zeros.  Valid for levels below 2^48
+ * bits (superblock indices are 32-bit); longer levels return
+ * PWMHIP_ERR_UNSUPPORTED.  The reference's directory is valid below 2^32-1
+ * bits only (bin_rank_popcnt.hpp:54-86, popcnt_traits.hpp:16-20). */
 typedef struct pwmhip_rs pwmhip_rs;
 int pwmhip_rs_build(pwmhip_ctx* ctx, const pwmhip_ws* ws, pwmhip_rs** out);
 int pwmhip_rs_free(pwmhip_ctx* ctx, pwmhip_rs* rs);

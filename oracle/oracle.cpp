@@ -19,6 +19,7 @@
 #include <functional>
 #include <numeric>
 #include <queue>
+#include <random>
 #include <unordered_map>
 #include <vector>
 
@@ -428,6 +429,16 @@ void oracle_copy_bits8(uint8_t* dst, const uint8_t* src, uint64_t dst_off,
     const uint64_t s = src_off + i, d = dst_off + i;
     if ((src[s >> 3] >> (7 - (s & 7))) & 1) dst[d >> 3] |= uint8_t(1u << (7 - (d & 7)));
   }
+}
+
+// The bit vector of the reference's select test
+// (test/bin_rank_select_tests.cpp:39-62): n_words words drawn in order from
+// std::uniform_int_distribution<uint64_t>(0) over std::mt19937(seed).  Both
+// are the standard library's, so the same library gives the same words.
+void oracle_mt19937_words(uint64_t seed, uint64_t n_words, uint64_t* out) {
+  std::mt19937 gen(static_cast<std::mt19937::result_type>(seed));
+  std::uniform_int_distribution<uint64_t> dis(0);
+  for (uint64_t i = 0; i < n_words; ++i) out[i] = dis(gen);
 }
 
 // reduce_alphabet (util/alphabet_util.hpp:18-45): symbols renumbered in

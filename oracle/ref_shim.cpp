@@ -80,10 +80,11 @@ int codes_of(const wavelet_structure& ws, uint64_t* len, uint64_t* word, uint64_
 
 extern "C" {
 
-// Returns an opaque handle or nullptr for an unknown algorithm name.
+// Returns an opaque handle (an empty structure for an unknown algorithm
+// name), or nullptr if the reference threw (its allocations can).
 // *seconds receives the wall time of compute() alone (steady_clock).
 void* ref_build(const char* algo, int width, int is_tree, const void* text,
-                uint64_t n, uint64_t levels, double* seconds) {
+                uint64_t n, uint64_t levels, double* seconds) try {
   const std::string a(algo);
   const bool tree = is_tree != 0;
   auto t0 = std::chrono::steady_clock::now();
@@ -112,6 +113,8 @@ void* ref_build(const char* algo, int width, int is_tree, const void* text,
   h->tree = tree;
   h->huff = a.rfind("huff", 0) == 0;
   return h;
+} catch (...) {
+  return nullptr;
 }
 
 void ref_free(void* h) { delete static_cast<handle*>(h); }

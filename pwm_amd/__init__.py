@@ -302,7 +302,10 @@ def build(text, levels, algo="ppc", is_tree=False, ctx=None, check_range=False, 
     if stream:
         flags |= FLAG_STREAM
     ws = _WS()
-    if reuse is not None and reuse._ws is not None:
+    # a structure of a batch is a view into the batch's shared block, which
+    # its siblings still use: only a structure that owns its memory is
+    # taken over
+    if reuse is not None and reuse._ws is not None and getattr(reuse, "_block", None) is None:
         ws = reuse._ws
         reuse._ws = None
         flags |= FLAG_REUSE_OUTPUT

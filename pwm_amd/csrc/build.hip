@@ -2805,59 +2805,41 @@ namespace {
 // PWMHIP_GENERIC_K1=1 routes every pass through the generic local build
 // (for comparisons).
 bool small_k1_disabled() {
-  static const bool off = [] {
-    const char* e = std::getenv("PWMHIP_GENERIC_K1");
-    return e && e[0] == '1';
-  }();
-  return off;
+  const char* e = std::getenv("PWMHIP_GENERIC_K1");  // read at every call
+  return e && e[0] == '1';
 }
 
 // PWMHIP_CHUNK_RUNS=1: one run per chunk (no super-chunk assembly).
 bool super_disabled() {
-  static const bool off = [] {
-    const char* e = std::getenv("PWMHIP_CHUNK_RUNS");
-    return e && e[0] == '1';
-  }();
-  return off;
+  const char* e = std::getenv("PWMHIP_CHUNK_RUNS");  // read at every call
+  return e && e[0] == '1';
 }
 
 // PWMHIP_NARROW_GATHER=1: one destination word per lane in the stitch.
 bool wide_gather_disabled() {
-  static const bool off = [] {
-    const char* e = std::getenv("PWMHIP_NARROW_GATHER");
-    return e && e[0] == '1';
-  }();
-  return off;
+  const char* e = std::getenv("PWMHIP_NARROW_GATHER");  // read at every call
+  return e && e[0] == '1';
 }
 
 // 128-word steps per wave of the wide stitch (multiple of 4; PWMHIP_WIDE_STEPS).
 uint32_t wide_gather_steps() {
-  static const uint32_t v = [] {
-    const char* e = std::getenv("PWMHIP_WIDE_STEPS");
-    const long x = e ? std::atol(e) : 8;
-    return uint32_t(x >= 4 && x <= 256 ? (x + 3) / 4 * 4 : 8);
-  }();
-  return v;
+  const char* e = std::getenv("PWMHIP_WIDE_STEPS");  // read at every call
+  const long x = e ? std::atol(e) : 8;
+  return uint32_t(x >= 4 && x <= 256 ? (x + 3) / 4 * 4 : 8);
 }
 
 // PWMHIP_SCATTER_RANKED=1: the symbol reorder ranks 64 symbols per step
 // (k_scatter) instead of the per-lane counting sort.
 bool scatter_seg_disabled() {
-  static const bool off = [] {
-    const char* e = std::getenv("PWMHIP_SCATTER_RANKED");
-    return e && e[0] == '1';
-  }();
-  return off;
+  const char* e = std::getenv("PWMHIP_SCATTER_RANKED");  // read at every call
+  return e && e[0] == '1';
 }
 
 // Chunks per super-chunk unit of the plain builds: 16 (one workgroup of 16
 // waves per CU) or 8 (two workgroups of 8 per CU); PWMHIP_SUPER_WAVES.
 uint32_t super_waves() {
-  static const uint32_t v = [] {
-    const char* e = std::getenv("PWMHIP_SUPER_WAVES");
-    return e && std::atoi(e) == 8 ? 8u : 16u;
-  }();
-  return v;
+  const char* e = std::getenv("PWMHIP_SUPER_WAVES");  // read at every call
+  return e && std::atoi(e) == 8 ? 8u : 16u;
 }
 
 template <int G, int WAVES>
@@ -3631,8 +3613,9 @@ void build_plain_device(pwmhip_ctx* ctx, const void* d_text, uint64_t n, uint32_
     ctx->pinned = static_cast<uint64_t*>(hp);
     ctx->pinned_words = 128;
   }
-  PWM_HIP_TRY(hipMemcpyAsync(ctx->pinned, zeros_dev, L * 8, hipMemcpyDeviceToHost, s));
-  PWM_HIP_TRY(hipMemcpyAsync(ctx->pinned + 64, sym_or, 8, hipMemcpyDeviceToHost, s));
+  uint64_t* res = ctx->pinned;
+  PWM_HIP_TRY(hipMemcpyAsync(res, zeros_dev, L * 8, hipMemcpyDeviceToHost, s));
+  PWM_HIP_TRY(hipMemcpyAsync(res + 64, sym_or, 8, hipMemcpyDeviceToHost, s));
   PWM_HIP_TRY(hipStreamSynchronize(s));
   float ms = 0;
   PWM_HIP_TRY(hipEventElapsedTime(&ms, ctx->ev[0], ctx->ev[1]));
@@ -3643,12 +3626,12 @@ void build_plain_device(pwmhip_ctx* ctx, const void* d_text, uint64_t n, uint32_
   ctx->last_ms[2] = ms;
   PWM_HIP_TRY(hipEventElapsedTime(&ms, ctx->ev[0], ctx->ev[3]));
   ctx->last_ms[3] = ms;
-  if (check_range && L < 64 && (ctx->pinned[64] >> L) != 0)
+  if (check_range && L < 64 && (res[64] >> L) != 0)
     throw status_error{PWMHIP_ERR_SYMBOL_RANGE, "symbol >= 2^levels"};
   // trees above one pass: matrix levels with buckets moved into numeric order
   if (tree && multi) tree_order_timed(ctx, d_text, n, width, L, ws);
   ws->n_zeros = tree ? 0 : L;
-  for (uint32_t l = 0; l < L; ++l) ws->zeros[l] = tree ? 0 : ctx->pinned[l];
+  for (uint32_t l = 0; l < L; ++l) ws->zeros[l] = tree ? 0 : res[l];
 }
 
 // ---------------------------------------------------------------------------
@@ -3669,6 +3652,12 @@ const BatchTune kTuneCandidatesUnits[] = {{1, 4}, {1, 8}, {1, 16}};
 void build_batch_device(pwmhip_ctx* ctx, const void* const* d_texts, const uint64_t* sizes, uint64_t T,
                         uint32_t width, uint32_t L, bool tree, bool check_range, pwmhip_ws* outs, void* reuse,
                         uint64_t reuse_bytes, uint32_t flags) {
+  struct ReuseGuard {  // an earlier block not taken over is freed on every exit
+    void*& p;
+    ~ReuseGuard() {
+      if (p) (void)hipFree(p);
+    }
+  } reuse_guard{reuse};
   hipStream_t s = ctx->stream;
   const uint32_t g = L;  // one pass (L <= kMaxGroup)
   const uint32_t jmax = g - 1;
@@ -3699,8 +3688,10 @@ void build_batch_device(pwmhip_ctx* ctx, const void* const* d_texts, const uint6
   if (reuse && reuse_bytes >= block_bytes) {
     block = reuse;
     owned_bytes = reuse_bytes;
+    reuse = nullptr;
   } else {
     if (reuse) (void)hipFree(reuse);
+    reuse = nullptr;
     PWM_HIP_TRY(hipMalloc(&block, block_bytes));
   }
   struct BlockGuard {  // frees the block unless the build completes
@@ -4008,3 +3999,4 @@ void build_batch_device(pwmhip_ctx* ctx, const void* const* d_texts, const uint6
 }
 
 }  // namespace pwm
+

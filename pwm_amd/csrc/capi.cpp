@@ -32,27 +32,7 @@ void arena::release() {
 
 namespace {
 
-int run_guarded(const char* /*what*/, auto&& fn) {
-  try {
-    fn();
-    return PWMHIP_OK;
-  } catch (const hip_failure& f) {
-    (void)hipGetLastError();
-    return f.status;
-  } catch (const status_error& e) {
-    return e.status;
-  } catch (const std::bad_alloc&) {
-    return PWMHIP_ERR_OOM;
-  } catch (...) {
-    return PWMHIP_ERR_HIP;
-  }
-}
-
-// zeros[levels-1] is never computed by these reference algorithms.
-bool zeros_last_is_zero(uint32_t algo) {
-  return algo == PWMHIP_ALGO_PPC || algo == PWMHIP_ALGO_PPC_SS || algo == PWMHIP_ALGO_PC_SS ||
-         algo == PWMHIP_ALGO_DD_PC_SS;
-}
+int run_guarded(const char* /*what*/, auto&& fn) { return guarded(fn); }
 
 bool is_huffman_algo(uint32_t algo) { return algo >= PWMHIP_ALGO_HUFF_PC && algo <= PWMHIP_ALGO_HUFF_DD_PC; }
 
@@ -142,26 +122,34 @@ int pwmhip_destroy(pwmhip_ctx* ctx) {
 int pwmhip_build(pwmhip_ctx* ctx, const void* text, uint64_t size, uint32_t word_width,
                  uint32_t levels, uint32_t algo, uint32_t flags, pwmhip_ws* out) {
   if (!ctx || !out) return PWMHIP_ERR_INVALID;
-  uint64_t* reuse_ptr = nullptr;
-  uint64_t reuse_bytes = 0;
-  if ((flags & PWMHIP_FLAG_REUSE_OUTPUT) && out->d_words) {
-    reuse_ptr = out->d_words;
-    reuse_bytes = out->alloc_bytes;
+  // An earlier result is taken over only when it owns its block: a view into
+  // a batch's block (alloc_bytes == 0) is left to the batch's owner.
+  const bool take = (flags & PWMHIP_FLAG_REUSE_OUTPUT) && out->d_words && out->alloc_bytes;
+  const bool huff = pwm::is_huffman_algo(algo);
+  const bool stream = (flags & PWMHIP_FLAG_STREAM) && !(flags & PWMHIP_FLAG_TEXT_ON_DEVICE) && size > 0;
+  int bad = PWMHIP_OK;
+  if (word_width != 1 && word_width != 2 && word_width != 4 && word_width != 8) bad = PWMHIP_ERR_INVALID;
+  else if (algo > PWMHIP_ALGO_HUFF_DD_PC) bad = PWMHIP_ERR_INVALID;
+  else if (!huff && (levels == 0 || levels > 8 * word_width || levels > PWMHIP_MAX_LEVELS)) bad = PWMHIP_ERR_INVALID;
+  else if (size > 0 && !text) bad = PWMHIP_ERR_INVALID;
+  else if (stream && huff) bad = PWMHIP_ERR_UNSUPPORTED;
+  if (bad != PWMHIP_OK) {
+    // rejected before anything is taken over: an earlier result stays the
+    // caller's (and valid); anything else is cleared
+    if (!take) std::memset(out, 0, sizeof(*out));
+    return bad;
+  }
+  uint64_t* reuse_ptr = take ? out->d_words : nullptr;
+  const uint64_t reuse_bytes = take ? out->alloc_bytes : 0;
+  if (flags & PWMHIP_FLAG_REUSE_OUTPUT) {
     std::free(out->code_len);
     std::free(out->code_word);
   }
   std::memset(out, 0, sizeof(*out));
-  if (word_width != 1 && word_width != 2 && word_width != 4 && word_width != 8) return PWMHIP_ERR_INVALID;
-  if (algo > PWMHIP_ALGO_HUFF_DD_PC) return PWMHIP_ERR_INVALID;
-  const bool huff = pwm::is_huffman_algo(algo);
   const bool tree = (flags & PWMHIP_FLAG_TREE) != 0;
   out->is_tree = tree;
   out->is_huffman = huff;
-  if (!huff && (levels == 0 || levels > 8 * word_width || levels > PWMHIP_MAX_LEVELS)) return PWMHIP_ERR_INVALID;
-  if (size > 0 && !text) return PWMHIP_ERR_INVALID;
-  const bool stream = (flags & PWMHIP_FLAG_STREAM) && !(flags & PWMHIP_FLAG_TEXT_ON_DEVICE) && size > 0;
-  if (stream && huff) return PWMHIP_ERR_UNSUPPORTED;
-  return pwm::run_guarded("build", [&] {
+  const int st = pwm::run_guarded("build", [&] {
     PWM_HIP_TRY(hipSetDevice(ctx->device));
     const void* d_text = text;
     if (size > 0 && !(flags & PWMHIP_FLAG_TEXT_ON_DEVICE) && !stream) {
@@ -193,26 +181,46 @@ int pwmhip_build(pwmhip_ctx* ctx, const void* text, uint64_t size, uint32_t word
       return;
     }
     std::vector<uint64_t> bits(levels, size);
-    pwm::alloc_structure(out, levels, bits.data(), reuse_ptr, reuse_bytes);
-    reuse_ptr = nullptr;
+    uint64_t* r = reuse_ptr;
+    reuse_ptr = nullptr;  // (alloc_structure takes it over or frees it)
+    pwm::alloc_structure(out, levels, bits.data(), r, reuse_bytes);
     try {
       pwm::build_plain_device(ctx, stream ? nullptr : d_text, size, word_width, levels, tree,
                               (flags & PWMHIP_FLAG_CHECK_RANGE) != 0, out, stream ? text : nullptr);
     } catch (...) {
       (void)hipFree(out->d_words);
       out->d_words = nullptr;
+      out->alloc_bytes = 0;
       throw;
     }
     if (!tree && pwm::zeros_last_is_zero(algo)) out->zeros[levels - 1] = 0;
   });
+  if (reuse_ptr) (void)hipFree(reuse_ptr);  // failed before the output took it over
+  if (st != PWMHIP_OK) {
+    // nothing half-built is handed back
+    if (out->d_words && out->alloc_bytes) (void)hipFree(out->d_words);
+    std::free(out->code_len);
+    std::free(out->code_word);
+    std::memset(out, 0, sizeof(*out));
+  }
+  return st;
 }
 
 int pwmhip_build_batch(pwmhip_ctx* ctx, const void* const* texts, const uint64_t* sizes, uint64_t count,
                        uint32_t word_width, uint32_t levels, uint32_t algo, uint32_t flags, pwmhip_ws* outs) {
   if (!ctx || (count && (!texts || !sizes || !outs))) return PWMHIP_ERR_INVALID;
+  // arguments first: a rejected call takes nothing over and frees nothing
+  if (word_width != 1 && word_width != 2 && word_width != 4 && word_width != 8) return PWMHIP_ERR_INVALID;
+  if (algo > PWMHIP_ALGO_HUFF_DD_PC) return PWMHIP_ERR_INVALID;
+  const bool huff = pwm::is_huffman_algo(algo);
+  const bool tree = (flags & PWMHIP_FLAG_TREE) != 0;
+  if (!huff && (levels == 0 || levels > 8 * word_width || levels > PWMHIP_MAX_LEVELS)) return PWMHIP_ERR_INVALID;
+  for (uint64_t t = 0; t < count; ++t)
+    if (sizes[t] > 0 && !texts[t]) return PWMHIP_ERR_INVALID;
+  if (count == 0) return PWMHIP_OK;
   void* reuse_ptr = nullptr;
   uint64_t reuse_bytes = 0;
-  if ((flags & PWMHIP_FLAG_REUSE_OUTPUT) && count && outs[0].d_words && outs[0].alloc_bytes) {
+  if ((flags & PWMHIP_FLAG_REUSE_OUTPUT) && outs[0].d_words && outs[0].alloc_bytes) {
     reuse_ptr = outs[0].d_words;
     reuse_bytes = outs[0].alloc_bytes;
   }
@@ -234,14 +242,6 @@ int pwmhip_build_batch(pwmhip_ctx* ctx, const void* const* texts, const uint64_t
     w.code_word = nullptr;
     w.alloc_bytes = 0;
   }
-  if (word_width != 1 && word_width != 2 && word_width != 4 && word_width != 8) return PWMHIP_ERR_INVALID;
-  if (algo > PWMHIP_ALGO_HUFF_DD_PC) return PWMHIP_ERR_INVALID;
-  const bool huff = pwm::is_huffman_algo(algo);
-  const bool tree = (flags & PWMHIP_FLAG_TREE) != 0;
-  if (!huff && (levels == 0 || levels > 8 * word_width || levels > PWMHIP_MAX_LEVELS)) return PWMHIP_ERR_INVALID;
-  for (uint64_t t = 0; t < count; ++t)
-    if (sizes[t] > 0 && !texts[t]) return PWMHIP_ERR_INVALID;
-  if (count == 0) return PWMHIP_OK;
   if (huff || levels > uint32_t(pwm::kMaxGroup)) {
     if (reuse_ptr) {
       (void)hipSetDevice(ctx->device);

@@ -432,11 +432,8 @@ struct HStitch {
 
 // PWMHIP_HUFF_RUN_STITCH=1: the run-major stitch for huff_naive layouts too.
 bool wide_gather_disabled_huff() {
-  static const bool off = [] {
-    const char* e = std::getenv("PWMHIP_HUFF_RUN_STITCH");
-    return e && e[0] == '1';
-  }();
-  return off;
+  const char* e = std::getenv("PWMHIP_HUFF_RUN_STITCH");  // read at every call
+  return e && e[0] == '1';
 }
 
 // Words of levels >= 1 that hold a run's first or last bit (shared with the
@@ -623,11 +620,8 @@ inline uint32_t rev(uint64_t x, uint64_t bits) {
 
 // PWMHIP_HUFF_CHUNKS=1: the general per-chunk Huffman kernels for every input.
 bool huffman_super_disabled() {
-  static const bool off = [] {
-    const char* e = std::getenv("PWMHIP_HUFF_CHUNKS");
-    return e && e[0] == '1';
-  }();
-  return off;
+  const char* e = std::getenv("PWMHIP_HUFF_CHUNKS");  // read at every call
+  return e && e[0] == '1';
 }
 
 // Byte texts, codes of 2..4 bits, node ranges that do not overlap: local

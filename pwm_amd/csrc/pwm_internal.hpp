@@ -4,6 +4,8 @@
 #include <hip/hip_runtime.h>
 
 #include <cstdint>
+#include <new>
+#include <stdexcept>
 #include <unordered_map>
 #include <utility>
 #include <cstdio>
@@ -35,6 +37,29 @@ struct status_error {
   int status;
   std::string what;
 };
+
+// Runs fn() and turns every exception into a status, so that nothing
+// crosses the C-ABI (or leaves a thread body): HIP failures keep their
+// status, host allocation failures (bad_alloc, a vector past max_size) are
+// out-of-memory, anything else is a runtime error.
+template <typename F>
+int guarded(F&& fn) noexcept {
+  try {
+    fn();
+    return PWMHIP_OK;
+  } catch (const hip_failure& f) {
+    (void)hipGetLastError();
+    return f.status;
+  } catch (const status_error& e) {
+    return e.status;
+  } catch (const std::bad_alloc&) {
+    return PWMHIP_ERR_OOM;
+  } catch (const std::length_error&) {
+    return PWMHIP_ERR_OOM;
+  } catch (...) {
+    return PWMHIP_ERR_HIP;
+  }
+}
 
 // ---- geometry ---------------------------------------------------------------
 // A chunk is the unit of the local build (one wave): kChunkWords 64-bit
@@ -137,6 +162,13 @@ const uint8_t* compress_table(pwmhip_ctx* ctx);
 void build_plain_device(pwmhip_ctx* ctx, const void* d_text, uint64_t n, uint32_t width,
                         uint32_t levels, bool tree, bool check_range, pwmhip_ws* ws,
                         const void* h_stream = nullptr);
+
+// zeros[levels-1] is never computed by these reference algorithms
+// (ppc.hpp:19-90, ppc_ss.hpp:19-70, pc_ss.hpp:13-33, wx_dd_pc_ss.hpp).
+inline bool zeros_last_is_zero(uint32_t algo) {
+  return algo == PWMHIP_ALGO_PPC || algo == PWMHIP_ALGO_PPC_SS || algo == PWMHIP_ALGO_PC_SS ||
+         algo == PWMHIP_ALGO_DD_PC_SS;
+}
 
 // Builds T texts (device pointers) with one launch per phase.  All
 // structures live in one allocation owned by outs[0] (alloc_bytes > 0, the

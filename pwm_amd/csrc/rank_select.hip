@@ -5,7 +5,7 @@
 // bin_rank_popcnt (bin_rank_popcnt.hpp:32-134) and bin_select_popcnt
 // (bin_select_popcnt.hpp:32-252).  Same semantics (rank1(i) = ones in [0, i),
 // select1(k) = 0-based position of the k-th one, select(0) = 0), different
-// layout, valid for levels of any length (the reference's uint32 L1 counts
+// layout, valid for levels below 2^48 bits (the reference's uint32 L1 counts
 // and 2^32-1-bit L0 blocks break at 2^32 bits, which every level of a 4 Gi-
 // symbol text reaches):
 //   superblock = 64 Ki bits : absolute uint64 count of ones before it
@@ -40,6 +40,9 @@ constexpr uint32_t kSbBits = 1u << 16;
 constexpr uint32_t kBlkBits = 512;
 constexpr uint32_t kBlkPerSb = kSbBits / kBlkBits;  // 128
 constexpr uint32_t kSample = 8192;
+// select samples hold uint32 superblock indices and the block-count kernel's
+// grid has one workgroup per superblock: levels stay below 2^32 superblocks
+constexpr uint64_t kMaxLevelBits = uint64_t(1) << 48;
 
 struct LevelRS {
   const uint64_t* words;
@@ -410,9 +413,12 @@ int pwmhip_rs_build(pwmhip_ctx* ctx, const pwmhip_ws* ws, pwmhip_rs** out) {
   // the symbol recursions can walk
   if (ws->is_huffman == 1) return PWMHIP_ERR_UNSUPPORTED;
   const bool huff = ws->is_huffman != 0;
+  if (ws->levels > PWMHIP_MAX_LEVELS) return PWMHIP_ERR_INVALID;
+  for (uint64_t l = 0; l < ws->levels; ++l)
+    if (ws->level_bits[l] >= kMaxLevelBits) return PWMHIP_ERR_UNSUPPORTED;
   auto* rs = new (std::nothrow) pwmhip_rs();
   if (!rs) return PWMHIP_ERR_OOM;
-  try {
+  const int st = pwm::guarded([&] {
     PWM_HIP_TRY(hipSetDevice(ctx->device));
     const uint64_t L = ws->levels;
     rs->levels = L;
@@ -504,11 +510,11 @@ int pwmhip_rs_build(pwmhip_ctx* ctx, const pwmhip_ws* ws, pwmhip_rs** out) {
     PWM_HIP_TRY(hipMemcpyAsync(rs->d_desc, desc.data(), L * sizeof(LevelRS), hipMemcpyHostToDevice, ctx->stream));
     PWM_HIP_TRY(hipMemcpyAsync(d_zeros, rs->true_zeros, L * 8, hipMemcpyHostToDevice, ctx->stream));
     PWM_HIP_TRY(hipStreamSynchronize(ctx->stream));
-  } catch (const hip_failure& f) {
-    (void)hipGetLastError();
+  });
+  if (st != PWMHIP_OK) {
     if (rs->mem) (void)hipFree(rs->mem);
     delete rs;
-    return f.status;
+    return st;
   }
   *out = rs;
   return PWMHIP_OK;
@@ -531,9 +537,9 @@ namespace {
 template <typename F>
 int run_query(pwmhip_ctx* ctx, uint64_t n, uint32_t flags, const void* const* ins, const size_t* isz, int nin,
               uint64_t* result, F&& launch) {
-  try {
+  return pwm::guarded([&] {
     PWM_HIP_TRY(hipSetDevice(ctx->device));
-    if (n == 0) return PWMHIP_OK;
+    if (n == 0) return;
     const bool on_dev = (flags & PWMHIP_FLAG_TEXT_ON_DEVICE) != 0;
     std::vector<const void*> d_in(nin);
     uint64_t* d_res = result;
@@ -555,13 +561,7 @@ int run_query(pwmhip_ctx* ctx, uint64_t n, uint32_t flags, const void* const* in
     PWM_HIP_TRY(hipGetLastError());
     if (!on_dev) PWM_HIP_TRY(hipMemcpyAsync(result, d_res, n * 8, hipMemcpyDeviceToHost, ctx->stream));
     PWM_HIP_TRY(hipStreamSynchronize(ctx->stream));
-  } catch (const hip_failure& f) {
-    (void)hipGetLastError();
-    return f.status;
-  } catch (const status_error& e) {
-    return e.status;
-  }
-  return PWMHIP_OK;
+  });
 }
 
 }  // namespace

@@ -120,6 +120,7 @@ struct pwmhip_server {
   void worker(Worker* w);
   void run_build(Worker* w, Job& job);
   void run_queries(Worker* w, Job& job);
+  void fail_job(const Job& job, int st);  // under mu
   void drop_index(uint64_t id);  // under mu
 };
 
@@ -402,6 +403,25 @@ void pwmhip_server::run_queries(Worker* w, Job& job) {
   }
 }
 
+// A job whose host bookkeeping threw (an allocation): requests it left
+// pending fail with `st`.  Under mu.
+void pwmhip_server::fail_job(const Job& job, int st) {
+  for (uint64_t id : job.builds) {
+    auto it = indices.find(id);
+    if (it == indices.end() || it->second.status != 1) continue;
+    it->second.status = st;
+    builds_failed++;
+    charge(-int64_t(it->second.est));
+  }
+  for (const auto& q : job.queries) {
+    auto qs = query_status.find(q.id);
+    if (qs == query_status.end() || qs->second != 1) continue;
+    qs->second = st;
+    auto ix = indices.find(q.index);
+    if (ix != indices.end() && ix->second.queries_pending) ix->second.queries_pending--;
+  }
+}
+
 void pwmhip_server::worker(Worker* w) {
   std::unique_lock<std::mutex> lk(mu);
   for (;;) {
@@ -409,9 +429,12 @@ void pwmhip_server::worker(Worker* w) {
     if (!w->busy) return;
     Job job = std::move(w->job);
     lk.unlock();
-    if (job.kind == Job::BUILD) run_build(w, job);
-    else run_queries(w, job);
+    const int st = pwm::guarded([&] {
+      if (job.kind == Job::BUILD) run_build(w, job);
+      else run_queries(w, job);
+    });
     lk.lock();
+    if (st != PWMHIP_OK) fail_job(job, st);
     w->busy = false;
     cv_sched.notify_all();
     cv_done.notify_all();
@@ -439,19 +462,37 @@ int pwmhip_server_create(const pwmhip_server_config* cfg, pwmhip_server** out) {
     delete s;
     return PWMHIP_ERR_BUDGET;
   }
-  for (uint32_t i = 0; i < s->cfg.workers; ++i) {
-    auto w = std::make_unique<pwmhip_server::Worker>();
-    const int st = pwmhip_create(&w->ctx, s->cfg.device);
-    if (st != PWMHIP_OK) {
-      for (auto& x : s->workers) pwmhip_destroy(x->ctx);
-      delete s;
-      return st;
+  int st = pwm::guarded([&] {
+    for (uint32_t i = 0; i < s->cfg.workers; ++i) {
+      auto w = std::make_unique<pwmhip_server::Worker>();
+      const int cst = pwmhip_create(&w->ctx, s->cfg.device);
+      if (cst != PWMHIP_OK) throw pwm::status_error{cst, "create"};
+      s->workers.push_back(std::move(w));
     }
-    s->workers.push_back(std::move(w));
+  });
+  if (st != PWMHIP_OK) {
+    for (auto& x : s->workers) pwmhip_destroy(x->ctx);
+    delete s;
+    return st;
   }
   s->charge(int64_t(s->cfg.workers) * int64_t(s->allowance));
-  for (auto& w : s->workers) w->th = std::thread(&pwmhip_server::worker, s, w.get());
-  s->sched = std::thread(&pwmhip_server::scheduler, s);
+  // threads: a failed start stops and joins the ones already running
+  st = pwm::guarded([&] {
+    for (auto& w : s->workers) w->th = std::thread(&pwmhip_server::worker, s, w.get());
+    s->sched = std::thread(&pwmhip_server::scheduler, s);
+  });
+  if (st != PWMHIP_OK) {
+    {
+      std::lock_guard<std::mutex> g(s->mu);
+      s->exiting = true;
+    }
+    s->cv_work.notify_all();
+    for (auto& w : s->workers)
+      if (w->th.joinable()) w->th.join();
+    for (auto& w : s->workers) pwmhip_destroy(w->ctx);
+    delete s;
+    return st;
+  }
   *out = s;
   return PWMHIP_OK;
 }
@@ -468,9 +509,7 @@ int pwmhip_server_destroy(pwmhip_server* s) {
   for (auto& w : s->workers) w->th.join();
   {
     std::lock_guard<std::mutex> g(s->mu);
-    std::vector<uint64_t> ids;
-    for (auto& kv : s->indices) ids.push_back(kv.first);
-    for (uint64_t id : ids) s->drop_index(id);
+    while (!s->indices.empty()) s->drop_index(s->indices.begin()->first);
   }
   for (auto& w : s->workers) pwmhip_destroy(w->ctx);
   delete s;
@@ -493,18 +532,26 @@ int pwmhip_server_build(pwmhip_server* s, const void* text, uint64_t size, uint3
   x.text = text;
   x.est = output_bytes(size, levels) + rs_bytes(size, levels) + 4096;
   x.t_submit = clk::now();
-  {
+  const int st = pwm::guarded([&] {
     std::lock_guard<std::mutex> g(s->mu);
     const uint64_t fixed = uint64_t(s->cfg.workers) * s->allowance;
     const uint64_t chunks = (size + kChunk - 1) / kChunk;
     if (fixed + x.est > s->cfg.hbm_budget_bytes ||
         (levels <= pwm::kMaxGroup && scratch_bytes(chunks, 1, size * word_width, levels) > s->allowance &&
          size * word_width > s->cfg.max_batch_bytes))
-      return PWMHIP_ERR_BUDGET;
-    *index_id = s->next_id++;
-    s->indices.emplace(*index_id, x);
-    s->pending_builds.push_back(*index_id);
-  }
+      throw pwm::status_error{PWMHIP_ERR_BUDGET, "build"};
+    const uint64_t id = s->next_id;
+    s->indices.emplace(id, x);
+    try {
+      s->pending_builds.push_back(id);
+    } catch (...) {
+      s->indices.erase(id);
+      throw;
+    }
+    s->next_id++;
+    *index_id = id;
+  });
+  if (st != PWMHIP_OK) return st;
   s->cv_sched.notify_all();
   return PWMHIP_OK;
 }
@@ -520,22 +567,31 @@ int pwmhip_server_query(pwmhip_server* s, uint64_t index_id, const uint32_t* op,
   q.n = n;
   q.result = result;
   q.t_submit = clk::now();
-  {
+  bool queued = false;
+  const int st = pwm::guarded([&] {
     std::lock_guard<std::mutex> g(s->mu);
     auto it = s->indices.find(index_id);
-    if (it == s->indices.end() || it->second.release_requested) return PWMHIP_ERR_INVALID;
-    if (n > s->cfg.max_query_batch) return PWMHIP_ERR_BUDGET;
-    q.id = s->next_id++;
-    *request_id = q.id;
+    if (it == s->indices.end() || it->second.release_requested) throw pwm::status_error{PWMHIP_ERR_INVALID, "query"};
+    if (n > s->cfg.max_query_batch) throw pwm::status_error{PWMHIP_ERR_BUDGET, "query"};
+    q.id = s->next_id;
     if (n == 0) {
       s->query_status[q.id] = PWMHIP_OK;
-      return PWMHIP_OK;
+    } else {
+      s->query_status[q.id] = 1;
+      try {
+        s->pending_queries.push_back(q);
+      } catch (...) {
+        s->query_status.erase(q.id);
+        throw;
+      }
+      it->second.queries_pending++;
+      queued = true;
     }
-    it->second.queries_pending++;
-    s->query_status[q.id] = 1;
-    s->pending_queries.push_back(q);
-  }
-  s->cv_sched.notify_all();
+    s->next_id++;
+    *request_id = q.id;
+  });
+  if (st != PWMHIP_OK) return st;
+  if (queued) s->cv_sched.notify_all();
   return PWMHIP_OK;
 }
 
@@ -617,6 +673,7 @@ int pwmhip_server_stats_get(pwmhip_server* s, pwmhip_server_stats* out) {
   if (!s || !out) return PWMHIP_ERR_INVALID;
   std::lock_guard<std::mutex> g(s->mu);
   std::memset(out, 0, sizeof(*out));
+  return pwm::guarded([&] {
   auto pct = [](std::vector<double> v, double p) {
     if (v.empty()) return 0.0;
     std::sort(v.begin(), v.end());
@@ -636,7 +693,7 @@ int pwmhip_server_stats_get(pwmhip_server* s, pwmhip_server_stats* out) {
   out->bytes_in_use = s->in_use;
   out->bytes_peak = s->peak;
   out->mean_texts_per_launch = s->build_launches ? double(s->texts_launched) / double(s->build_launches) : 0.0;
-  return PWMHIP_OK;
+  });
 }
 
 }  // extern "C"

@@ -144,10 +144,27 @@ int main() {
       sizes.push_back(t.size());
     }
     auto batch = pwm_hip::compute_batch<uint8_t>(ptrs, sizes, 3, PWMHIP_ALGO_PPC, false);
-    auto first = std::move(batch[0]);
-    batch.clear();  // the block stays alive through `first`
-    auto last = pwm_hip::wx_hip_ppc<uint8_t, false>::compute(texts[0].data(), texts[0].size(), 3);
-    EXPECT(first.levels() == last.levels(), "batch[0] levels");
+    // every structure against the oracle, words and zeros (ppc: zeros[2] = 0)
+    auto check_one = [&](const pwm_hip::wavelet_structure& ws, size_t i) {
+      const uint64_t n = texts[i].size();
+      if (n == 0) {
+        EXPECT(ws.levels() == 0, "batch[%zu]: empty text", i);
+        return;
+      }
+      const uint64_t W = (n + 63) / 64;
+      std::vector<uint64_t> ref(W * 3), z(3);
+      oracle_wx_naive(texts[i].data(), n, 1, 3, 0, ref.data(), z.data());
+      EXPECT(ws.levels() == 3, "batch[%zu] levels", i);
+      for (uint64_t l = 0; l < 3 && l < ws.levels(); ++l)
+        EXPECT(std::memcmp(ws.bvs()[l].data(), ref.data() + l * W, W * 8) == 0, "batch[%zu] level %llu", i,
+               (unsigned long long)l);
+      const auto zz = ws.zeros();
+      EXPECT(zz.size() == 3 && zz[0] == z[0] && zz[1] == z[1] && zz[2] == 0, "batch[%zu] zeros", i);
+    };
+    for (size_t i = 0; i < batch.size(); ++i) check_one(batch[i], i);
+    auto last = std::move(batch.back());
+    batch.clear();  // the block stays alive through `last`
+    check_one(last, texts.size() - 1);
   }
 
   // errors surface as exceptions with the C-ABI status

@@ -3,6 +3,7 @@
 `make -C oracle ref`).  Run in the build container:
 
     OMP_NUM_THREADS=4 python tests/golden/make_golden.py
+    OMP_NUM_THREADS=1 python tests/golden/make_golden.py --huff-dd
 
 Fixtures are data only: inputs plus the reference's outputs (level words,
 zeros, Huffman codes).  The corner-case strings below are the test data of the
@@ -65,6 +66,27 @@ def cases():
     return out
 
 
+def huff_dd_main():
+    """golden_huff_dd.npz: wx_huff_dd_pc (wx_huff_dd_pc.hpp:17-32, huff_dd.hpp:
+    29-160) on every text of golden_huff.npz.  Its output depends on the
+    OpenMP thread count (one domain per thread); the one-thread result is the
+    one the library reproduces, so this must run with OMP_NUM_THREADS=1."""
+    assert os.environ.get("OMP_NUM_THREADS") == "1", "run with OMP_NUM_THREADS=1"
+    z = np.load(os.path.join(HERE, "golden_huff.npz"))
+    out = {}
+    for k in sorted(x for x in z.files if x.endswith("_text")):
+        key = k[: -len("_text")]
+        t = z[k]
+        r, _ = O.ref_build("huff_dd_pc", t, 0, key.endswith("_wt"))
+        out[key + "_huff_dd_pc_words"] = np.concatenate(r.levels) if r.levels else np.zeros(0, np.uint64)
+        out[key + "_huff_dd_pc_bits"] = np.array(r.bits, np.uint64)
+        out[key + "_huff_dd_pc_zeros"] = np.array(r.zeros, np.uint64)
+        out[key + "_huff_dd_pc_codelen"] = r.codes[0]
+        out[key + "_huff_dd_pc_codeword"] = r.codes[1]
+    np.savez_compressed(os.path.join(HERE, "golden_huff_dd.npz"), **out)
+    print("huff_dd_pc cases:", len(out) // 5)
+
+
 def main():
     assert O.have_ref(), "build the reference shim first: make -C oracle ref"
     plain = {}
@@ -124,4 +146,7 @@ def main():
 
 
 if __name__ == "__main__":
-    main()
+    if "--huff-dd" in sys.argv:
+        huff_dd_main()
+    else:
+        main()

@@ -24,21 +24,25 @@ def plain_cases():
 
 
 def huff_cases():
+    """Huffman cases; algos holds huff_pc, huff_naive, huff_ps, huff_pc_ss and
+    huff_dd_pc (the last written by the reference at one OpenMP thread)."""
     z = np.load(os.path.join(HERE, "golden_huff.npz"))
+    zd = np.load(os.path.join(HERE, "golden_huff_dd.npz"))
     keys = sorted({k.rsplit("_text", 1)[0] for k in z.files if k.endswith("_text")})
     out = []
     for k in keys:
         t = z[k + "_text"]
         d = dict(key=k, text=t, tree=k.endswith("_wt"), algos={})
-        for a in HUFF_ALGOS:
-            bits = [int(b) for b in z[k + "_" + a + "_bits"]]
-            words = z[k + "_" + a + "_words"]
+        for a, src in [(a, z) for a in HUFF_ALGOS] + [("huff_dd_pc", zd)]:
+            z_ = src
+            bits = [int(b) for b in z_[k + "_" + a + "_bits"]]
+            words = z_[k + "_" + a + "_words"]
             lv, off = [], 0
             for b in bits:
                 nw = (b + 63) // 64
                 lv.append(words[off:off + nw])
                 off += nw
-            d["algos"][a] = dict(bvs=lv, bits=bits, zeros=z[k + "_" + a + "_zeros"],
-                                 codes=(z[k + "_" + a + "_codelen"], z[k + "_" + a + "_codeword"]))
+            d["algos"][a] = dict(bvs=lv, bits=bits, zeros=z_[k + "_" + a + "_zeros"],
+                                 codes=(z_[k + "_" + a + "_codelen"], z_[k + "_" + a + "_codeword"]))
         out.append(d)
     return out

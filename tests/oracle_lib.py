@@ -45,6 +45,7 @@ def oracle():
         lib.oracle_copy_bits8.argtypes = [vp, vp, ctypes.c_uint64, ctypes.c_uint64, ctypes.c_uint64]
         lib.oracle_reduce_alphabet.argtypes = [vp, ctypes.c_uint64, ctypes.c_uint32]
         lib.oracle_reduce_alphabet.restype = ctypes.c_uint64
+        lib.oracle_mt19937_words.argtypes = [ctypes.c_uint64, ctypes.c_uint64, vp]
         _oracle = lib
     return _oracle
 
@@ -160,6 +161,8 @@ def ref_build(algo, text, levels, is_tree):
     secs = ctypes.c_double(0)
     h = lib.ref_build(algo.encode(), text.itemsize, int(is_tree), _ptr(text), text.size, levels,
                       ctypes.byref(secs))
+    if not h:
+        raise RuntimeError("compiled reference wx_%s::compute threw" % algo)
     try:
         L = lib.ref_levels(h)
         lv, bits = [], []
@@ -189,3 +192,11 @@ def reduce_alphabet(text):
     t = np.array(text, copy=True)
     levels = oracle().oracle_reduce_alphabet(_ptr(t), t.size, t.itemsize)
     return t, int(levels)
+
+
+def mt19937_words(seed, n_words):
+    """Words of the reference's select test: uniform_int_distribution<uint64_t>
+    over std::mt19937(seed) (test/bin_rank_select_tests.cpp:42-48)."""
+    out = np.zeros(max(1, n_words), np.uint64)
+    oracle().oracle_mt19937_words(seed, n_words, _ptr(out))
+    return out[:n_words]

@@ -57,18 +57,74 @@ def test_batch_many_uneven_dna_texts():
     offs = np.concatenate([[0], np.cumsum(sizes)])
     texts = [big[offs[i]:offs[i + 1]] for i in range(len(sizes))]
     got = pwm_amd.build_batch(texts, 3, "ppc", False)
-    sel = list(range(0, 2000, 37)) + [int(np.argmax(sizes))]
-    for i in sel:
+    # every text against the oracle: levels and zeros (ppc leaves zeros[L-1] at 0)
+    for i, t in enumerate(texts):
+        assert got[i].levels() == (3 if t.size else 0), i
+        if not t.size:
+            continue
+        o = O.naive(t, 3, False)
+        for l, (x, y) in enumerate(zip(got[i].bvs(), o.levels)):
+            assert np.array_equal(x, y), (i, l)
+        z = o.zeros.copy()
+        z[-1] = 0
+        assert np.array_equal(got[i].zeros(), z), i
+    for i in list(range(0, 2000, 37)) + [int(np.argmax(sizes))]:
         ref = pwm_amd.build(texts[i], 3, "ppc", False)
         _same(got[i], ref, i)
         ref.free()
-    # zero counts of all texts against the oracle's definition
-    for i in range(0, 2000, 11):
-        t = texts[i]
-        if t.size:
-            z = got[i].zeros()
-            assert int(z[0]) == int(np.count_nonzero((t >> 2) & 1 == 0))
-            assert int(z[2]) == 0  # ppc leaves zeros[L-1] at 0
+
+
+@pytest.mark.gpu
+def test_batch_dna_lognormal_texts_vs_oracle():
+    """The many-texts workload's length law (lognormal, median 64 KiB, sigma
+    1.5, clipped to [1 KiB, 64 MiB]) over the DNA-like source, 250 texts, each
+    structure against the oracle; also with graph replay and autotuning."""
+    rng = np.random.default_rng(0xC4)
+    sizes = np.clip(rng.lognormal(np.log(65536), 1.5, 250), 1024, 64 << 20).astype(np.int64)
+    big = pwm_amd.generate("dna", 0xC4, int(sizes.sum()))
+    offs = np.concatenate([[0], np.cumsum(sizes)])
+    texts = [big[offs[i]:offs[i + 1]] for i in range(len(sizes))]
+    ctx = pwm_amd.Context(0)
+    try:
+        prev = None
+        for rep in range(3):
+            got = pwm_amd.build_batch(texts, 3, "ppc", False, ctx=ctx, graph=rep > 0, autotune=rep > 0, reuse=prev)
+            for i, t in enumerate(texts):
+                o = O.naive(t, 3, False)
+                for l, (x, y) in enumerate(zip(got[i].bvs(), o.levels)):
+                    assert np.array_equal(x, y), (rep, i, l)
+                z = o.zeros.copy()
+                z[-1] = 0
+                assert np.array_equal(got[i].zeros(), z), (rep, i)
+            prev = got
+        del got, prev
+    finally:
+        ctx.close()
+
+
+@pytest.mark.gpu
+def test_reuse_of_batch_views_is_refused():
+    """build(reuse=<a structure of a batch>) must not take over (or free) the
+    batch's shared block: the new structure gets its own memory and every
+    structure of the batch stays intact."""
+    rng = np.random.default_rng(31)
+    texts = _texts(rng, [5000, 70000, 4096 * 3 + 5, 200], 1, 3)
+    res = pwm_amd.build_batch(texts, 3, "pps", False)
+    new_texts = _texts(rng, [9000, 123457], 1, 3)
+    a = pwm_amd.build(new_texts[0], 3, "pps", False, reuse=res[0])
+    b = pwm_amd.build(new_texts[1], 3, "pps", False, reuse=res[1])
+    for ws, t in ((a, new_texts[0]), (b, new_texts[1])):
+        o = O.naive(t, 3, False)
+        for x, y in zip(ws.bvs(), o.levels):
+            assert np.array_equal(x, y)
+    for i, t in enumerate(texts):
+        res[i]._bvs = None  # download again
+        o = O.naive(t, 3, False)
+        for l, (x, y) in enumerate(zip(res[i].bvs(), o.levels)):
+            assert np.array_equal(x, y), (i, l)
+    a.free()
+    b.free()
+    del res
 
 
 @pytest.mark.gpu
@@ -95,12 +151,22 @@ def test_batch_fallback_shapes_and_errors():
     for i, t in enumerate(texts):
         ref = pwm_amd.build(t, 12, "pc", False)
         _same(got[i], ref, ("L12", i))
+        if t.size:
+            o = O.naive(t, 12, False)
+            for l, (x, y) in enumerate(zip(got[i].bvs(), o.levels)):
+                assert np.array_equal(x, y), ("L12", i, l)
+            assert np.array_equal(got[i].zeros(), o.zeros)
     dna = [pwm_amd.generate("dna", 5 + i, 20000 + 17 * i) for i in range(3)]
     got = pwm_amd.build_batch(dna, 0, "huff_naive", True)
     for i, t in enumerate(dna):
         ref = pwm_amd.build(t, 0, "huff_naive", True)
         _same(got[i], ref, ("huff", i))
         assert np.array_equal(got[i].codes()[0], ref.codes()[0])
+        o = O.huff_naive(t, True)
+        assert [got[i].level_bit_size(l) for l in range(got[i].levels())] == o.bits
+        for l, (x, y) in enumerate(zip(got[i].bvs(), o.levels)):
+            assert np.array_equal(x, y), ("huff", i, l)
+        assert np.array_equal(got[i].codes()[0], o.codes[0]) and np.array_equal(got[i].codes()[1], o.codes[1])
     with pytest.raises(pwm_amd.PwmError) as e:
         pwm_amd.build_batch([np.array([1, 2], np.uint8), np.array([9], np.uint8)], 3, "ppc", check_range=True)
     assert e.value.status == pwm_amd.ERR_SYMBOL_RANGE
@@ -127,7 +193,8 @@ def test_batch_graph_replay_and_autotune():
             eager = pwm_amd.build_batch(texts, 3, "ppc", tree, ctx=ctx)
             for i in list(range(0, len(texts), 29)) + [int(np.argmax(sizes))]:
                 _same(got[i], eager[i], (rep, i))
-                if 0 < texts[i].size < 20000:
+            for i in range(len(texts)):
+                if texts[i].size:
                     o = O.naive(texts[i], 3, tree)
                     for l, (x, y) in enumerate(zip(got[i].bvs(), o.levels)):
                         assert np.array_equal(x, y), (rep, i, l)

@@ -22,7 +22,7 @@ def _same(ws, levels, bits, key):
 def test_huffman_golden():
     for c in golden_data.huff_cases():
         t, tree = c["text"], c["tree"]
-        for algo in ("huff_pc", "huff_naive", "huff_ps", "huff_pc_ss"):
+        for algo in ("huff_pc", "huff_naive", "huff_ps", "huff_pc_ss", "huff_dd_pc"):
             g = c["algos"][algo]
             ws = pwm_amd.build(t, 0, algo, tree)
             _same(ws, g["bvs"], g["bits"], (c["key"], algo))

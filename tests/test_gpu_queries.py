@@ -151,3 +151,93 @@ def test_huffman_symbol_queries_brute_force(algo):
                 qs.free()
                 ws.free()
     assert answered >= (8 if algo != "huff_pc" else 1)
+
+
+@pytest.mark.gpu
+def test_reference_select_test_at_its_size():
+    """test/bin_rank_select_tests.cpp:39-62 at its own size: 2^27 bits drawn
+    from std::mt19937(44227); for every position i below 2^27/100 the
+    reference asserts select1(ones so far) == i at a one and
+    select0(i - ones + 1) == i at a zero.  The same loop runs here, then
+    random rank / select over the whole vector against brute force."""
+    import torch
+    nbits = 1 << 27
+    words = O.mt19937_words(44227, nbits // 64)
+    bits = np.unpackbits(words.byteswap().view(np.uint8))  # MSB-first bit order
+    ws = pwm_amd.build(torch.from_numpy(bits).to("cuda:0"), 1, "ppc")
+    assert np.array_equal(ws.bvs()[0], words)
+    qs = pwm_amd.QuerySupport(ws)
+    m = nbits // 100
+    pre = bits[:m].astype(np.int64)
+    ones = np.flatnonzero(pre)
+    zeros = np.flatnonzero(pre == 0)
+    ops = np.concatenate([np.full(ones.size, 2), np.full(zeros.size, 3)]).astype(np.uint32)
+    args = np.concatenate([np.arange(1, ones.size + 1), np.arange(1, zeros.size + 1)]).astype(np.uint64)
+    got = qs.bv_query(np.zeros(ops.size, np.uint32), ops, args)
+    assert np.array_equal(got, np.concatenate([ones, zeros]).astype(np.uint64))
+    # the whole vector: random rank0/1 and select0/1
+    cum = np.concatenate([[0], np.cumsum(bits, dtype=np.int64)])
+    all_ones = np.flatnonzero(bits)
+    all_zeros = np.flatnonzero(bits == 0)
+    rng = np.random.default_rng(44227)
+    q = 1 << 20
+    i = rng.integers(0, nbits + 1, q)
+    k1 = rng.integers(1, all_ones.size + 1, q)
+    k0 = rng.integers(1, all_zeros.size + 1, q)
+    ops = np.repeat(np.arange(4, dtype=np.uint32), q)
+    args = np.concatenate([i, i, k1, k0]).astype(np.uint64)
+    want = np.concatenate([cum[i], i - cum[i], all_ones[k1 - 1], all_zeros[k0 - 1]]).astype(np.uint64)
+    got = qs.bv_query(np.zeros(ops.size, np.uint32), ops, args)
+    assert np.array_equal(got, want)
+    qs.free()
+    ws.free()
+
+
+@pytest.mark.gpu
+def test_rank_select_level_past_2_32():
+    """A level of 2^32 + 4097 bits (a one-level build of a periodic 0/1 text
+    made on the device): rank0/1 and select0/1 at positions on both sides of
+    2^32 against the closed forms of a periodic bit string (period P bits
+    with C ones: rank1(i) = (i // P) C + ones in the first i % P bits;
+    select1(k) = ((k-1) // C) P + position of the ((k-1) % C + 1)-th one)."""
+    import torch
+    n = (1 << 32) + 4097
+    P = 100003
+    rng = np.random.default_rng(2 ** 32 + 4097)
+    base = rng.integers(0, 2, P).astype(np.uint8)
+    d = torch.from_numpy(base).to("cuda:0").repeat((n + P - 1) // P)[:n]
+    ws = pwm_amd.build(d, 1, "ppc")
+    assert ws.level_bit_size(0) == n
+    # level words at sampled indices against the period
+    nw = (n + 63) // 64
+    lvl = ws.bvs()[0]
+    widx = np.unique(np.concatenate([rng.integers(0, nw, 2000), np.arange((1 << 26) - 4, (1 << 26) + 4),
+                                     [nw - 1]]))
+    pos = widx[:, None] * 64 + np.arange(64)[None, :]
+    expb = np.where(pos < n, base[pos % P], 0).astype(np.uint8)
+    expw = np.packbits(expb, axis=1).view(">u8").ravel().astype(np.uint64)
+    assert np.array_equal(lvl[widx], expw)
+    del d
+    qs = pwm_amd.QuerySupport(ws)
+    C = int(base.sum())
+    pre = np.concatenate([[0], np.cumsum(base, dtype=np.int64)])
+    one_pos = np.flatnonzero(base)
+    zero_pos = np.flatnonzero(base == 0)
+    total1 = (n // P) * C + int(pre[n % P])
+    total0 = n - total1
+    i = np.unique(np.concatenate([rng.integers(0, n + 1, 20000), rng.integers(1 << 32, n + 1, 5000),
+                                  np.arange((1 << 32) - 600, (1 << 32) + 600), [0, n - 1, n]])).astype(np.int64)
+    r1 = (i // P) * C + pre[i % P]
+    k1 = np.unique(np.concatenate([rng.integers(1, total1 + 1, 20000), [1, total1]])).astype(np.int64)
+    s1 = ((k1 - 1) // C) * P + one_pos[(k1 - 1) % C]
+    Z = P - C
+    k0 = np.unique(np.concatenate([rng.integers(1, total0 + 1, 20000), [1, total0]])).astype(np.int64)
+    s0 = ((k0 - 1) // Z) * P + zero_pos[(k0 - 1) % Z]
+    ops = np.concatenate([np.zeros(i.size), np.ones(i.size), np.full(k1.size, 2), np.full(k0.size, 3)])
+    args = np.concatenate([i, i, k1, k0]).astype(np.uint64)
+    want = np.concatenate([r1, i - r1, s1, s0]).astype(np.uint64)
+    got = qs.bv_query(np.zeros(ops.size, np.uint32), ops.astype(np.uint32), args)
+    bad = np.flatnonzero(got != want)
+    assert bad.size == 0, (bad[:5], ops[bad[:5]], args[bad[:5]], got[bad[:5]], want[bad[:5]])
+    qs.free()
+    ws.free()

@@ -1,7 +1,8 @@
 """Streamed host texts (PWMHIP_FLAG_STREAM, the in-HBM analogue of
-wx_ppc_ie, external_memory/wx_ppc_ie.hpp:21-65): the structure equals the
-one built from a device-resident copy of the text, with slices small enough
-that the text spans many of them (ragged last slice included)."""
+wx_ppc_ie, external_memory/wx_ppc_ie.hpp:21-65), with slices small enough
+that the text spans many of them (ragged last slice included): every level
+and zero count equals the oracle's (wx_naive restated) and the structure
+built from a device-resident copy of the text."""
 import os
 
 import numpy as np
@@ -16,6 +17,18 @@ pwm_amd = pytest.importorskip("pwm_amd")
 def small_slices(monkeypatch):
     monkeypatch.setenv("PWMHIP_STREAM_SLICE_SYMS", str(3 * 65536))  # three super-chunks per slice
     yield
+
+
+def _oracle(ws, t, L, tree, algo, key):
+    o = O.naive(t, L, tree)
+    assert ws.levels() == len(o.levels), key
+    for l, (x, y) in enumerate(zip(ws.bvs(), o.levels)):
+        assert np.array_equal(x, y), (key, l)
+    if not tree:
+        z = o.zeros.copy()
+        if algo == "ppc":
+            z[-1] = 0  # wx_ppc never sets zeros[levels-1] (ppc.hpp:19-90)
+        assert np.array_equal(ws.zeros(), z), key
 
 
 def _same(a, b, key):
@@ -37,22 +50,21 @@ def test_stream_equals_resident(small_slices, w, L):
                 a = pwm_amd.build(t, L, algo, tree, stream=True)
                 b = pwm_amd.build(t, L, algo, tree)
                 _same(a, b, (w, L, n, tree, algo))
-                if n < 20000:
-                    o = O.naive(t, L, tree)
-                    for l, (x, y) in enumerate(zip(a.bvs(), o.levels)):
-                        assert np.array_equal(x, y), (w, L, n, tree, l)
+                _oracle(a, t, L, tree, algo, (w, L, n, tree, algo))
                 a.free()
                 b.free()
 
 
 @pytest.mark.gpu
 def test_stream_dna_and_unsupported_shapes(small_slices):
-    t = pwm_amd.generate("dna", 0x5E, 3_333_333)
-    a = pwm_amd.build(t, 3, "ppc", False, stream=True)
-    b = pwm_amd.build(t, 3, "ppc", False)
-    _same(a, b, "dna")
-    a.free()
-    b.free()
+    t = pwm_amd.generate("dna", 0x5E, 3_333_333)  # 17 slices, the last one ragged
+    for tree in (False, True):
+        a = pwm_amd.build(t, 3, "ppc", tree, stream=True)
+        b = pwm_amd.build(t, 3, "ppc", tree)
+        _same(a, b, ("dna", tree))
+        _oracle(a, t, 3, tree, "ppc", ("dna", tree))
+        a.free()
+        b.free()
     for levels, algo in ((8, "ppc"), (1, "ppc"), (0, "huff_pc")):
         with pytest.raises(pwm_amd.PwmError) as e:
             pwm_amd.build(t, levels, algo, False, stream=True)

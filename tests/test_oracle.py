@@ -93,6 +93,23 @@ def test_huffman_oracle_matches_reference_golden(huff):
             assert g2["bits"] == c["algos"]["huff_pc"]["bits"]
 
 
+def test_huff_dd_pc_one_thread_is_huff_pc(huff):
+    """The reference's wx_huff_dd_pc at one OpenMP thread (golden_huff_dd.npz)
+    equals huff_pc on every golden text: levels, sizes, zeros and codes --
+    the oracle's huff_pc restatement therefore pins HUFF_DD_PC too."""
+    for c in huff:
+        g, d = c["algos"]["huff_pc"], c["algos"]["huff_dd_pc"]
+        assert d["bits"] == g["bits"], c["key"]
+        for x, y in zip(d["bvs"], g["bvs"]):
+            assert np.array_equal(x, y), c["key"]
+        assert np.array_equal(d["zeros"], g["zeros"]), c["key"]
+        assert np.array_equal(d["codes"][0], g["codes"][0]) and np.array_equal(d["codes"][1], g["codes"][1])
+        o = O.huff_pc(c["text"], c["tree"])
+        for x, y in zip(o.levels, d["bvs"]):
+            assert np.array_equal(x, y), c["key"]
+        assert np.array_equal(o.zeros, d["zeros"]), c["key"]
+
+
 def test_huffman_naive_decodes_back(huff):
     for c in huff[::4]:
         t, tree = c["text"], c["tree"]
@@ -242,3 +259,14 @@ def test_reduce_alphabet_first_occurrence():
     assert list(r) == [0, 1, 0] and levels == 1
     r, levels = O.reduce_alphabet(np.array([7, 7], np.uint8))
     assert list(r) == [0, 0] and levels == 0
+
+
+def test_select_test_words_match_an_independent_mt19937():
+    """The words of the reference's select test (bin_rank_select_tests.cpp:
+    42-48) from the oracle equal numpy's own MT19937 seeded the same way
+    (init_genrand), two 32-bit outputs per word, high half first."""
+    bg = np.random.MT19937()
+    bg._legacy_seeding(44227)
+    raw = bg.random_raw(2 * 4096).astype(np.uint64)
+    want = (raw[0::2] << np.uint64(32)) | raw[1::2]
+    assert np.array_equal(O.mt19937_words(44227, 4096), want)
