@@ -126,6 +126,29 @@ int pwmhip_build(pwmhip_ctx* ctx, const void* text, uint64_t size, uint32_t word
 
 int pwmhip_ws_free(pwmhip_ctx* ctx, pwmhip_ws* ws);
 
+/* ---- allocation-free construction ---------------------------------------
+ * No counterpart in the reference.  For callers that cannot allocate device
+ * memory (code under HIP graph capture, tight serving loops, harnesses that
+ * forbid hipMalloc): the same build as pwmhip_build for plain algorithms
+ * (algo <= PWMHIP_ALGO_NAIVE), 2 to 4 levels, a device text, into the
+ * caller's device memory `d_out` of `out_words` 64-bit words, with the
+ * build's temporaries in a static device array of the library (128 MiB plus
+ * a 64 KiB table, held on every device the library runs on until the process
+ * ends).  It calls no hipMalloc, hipFree or other allocating routine.
+ * Results equal pwmhip_build's word for word; `out` describes them (level l
+ * at d_out + out->level_offset[l]; alloc_bytes = 0: pwmhip_ws_free frees
+ * nothing).  Needed: out_words >= levels * ((ceil(size/64) + 2 + 31) / 32 * 32).
+ * Flags: PWMHIP_FLAG_TREE, PWMHIP_FLAG_CHECK_RANGE (PWMHIP_FLAG_TEXT_ON_DEVICE
+ * is implied).  Texts whose temporaries exceed the static array (about 2^29
+ * symbols at 3 levels: (levels - 1) * size / 8 bytes plus per-64 Ki-symbol
+ * tables) return PWMHIP_ERR_BUDGET before any work; other shapes return
+ * PWMHIP_ERR_UNSUPPORTED.  Every call on one device shares its array, and
+ * each call leaves its contents behind: calls that overlap on one device
+ * (two contexts, two threads) race and must be serialised by the caller. */
+int pwmhip_build_noalloc(pwmhip_ctx* ctx, const void* d_text, uint64_t size, uint32_t word_width,
+                         uint32_t levels, uint32_t algo, uint32_t flags, uint64_t* d_out,
+                         uint64_t out_words, pwmhip_ws* out);
+
 /* Copies level `level` (ceil(level_bits/64) words) to host memory. */
 int pwmhip_ws_download(pwmhip_ctx* ctx, const pwmhip_ws* ws, uint64_t level, uint64_t* host_words);
 

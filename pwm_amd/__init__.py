@@ -100,6 +100,7 @@ def _load():
         "pwmhip_destroy": [vp],
         "pwmhip_device_count": [ctypes.POINTER(ctypes.c_int)],
         "pwmhip_build": [vp, vp, u64, u32, u32, u32, u32, ctypes.POINTER(_WS)],
+        "pwmhip_build_noalloc": [vp, vp, u64, u32, u32, u32, u32, vp, u64, ctypes.POINTER(_WS)],
         "pwmhip_ws_free": [vp, ctypes.POINTER(_WS)],
         "pwmhip_ws_download": [vp, ctypes.POINTER(_WS), u64, vp],
         "pwmhip_rs_build": [vp, ctypes.POINTER(_WS), ctypes.POINTER(vp)],

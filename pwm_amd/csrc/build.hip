@@ -42,6 +42,7 @@
 #include <cstring>
 #include <algorithm>
 #include <chrono>
+#include <mutex>
 
 namespace pwm {
 
@@ -3012,8 +3013,7 @@ void dispatch_k1(pwmhip_ctx* ctx, const K1Params& prm) {
 // ---------------------------------------------------------------------------
 // host orchestration
 // ---------------------------------------------------------------------------
-void alloc_structure(pwmhip_ws* ws, uint64_t levels, const uint64_t* bits, uint64_t* reuse,
-                     uint64_t reuse_bytes) {
+uint64_t layout_structure(pwmhip_ws* ws, uint64_t levels, const uint64_t* bits) {
   uint64_t off = 0;
   for (uint64_t l = 0; l < levels; ++l) {
     ws->level_offset[l] = off;
@@ -3021,7 +3021,12 @@ void alloc_structure(pwmhip_ws* ws, uint64_t levels, const uint64_t* bits, uint6
     off += level_stride(bits[l]);
   }
   ws->levels = levels;
-  ws->alloc_bytes = std::max<uint64_t>(off, 32) * 8;
+  return std::max<uint64_t>(off, 32);
+}
+
+void alloc_structure(pwmhip_ws* ws, uint64_t levels, const uint64_t* bits, uint64_t* reuse,
+                     uint64_t reuse_bytes) {
+  ws->alloc_bytes = layout_structure(ws, levels, bits) * 8;
   if (reuse) {
     if (reuse_bytes >= ws->alloc_bytes) {
       ws->d_words = reuse;
@@ -3383,7 +3388,10 @@ void stream_local_build(pwmhip_ctx* ctx, const void* h_text, uint64_t n, uint32_
 }  // namespace
 
 void build_plain_device(pwmhip_ctx* ctx, const void* d_text, uint64_t n, uint32_t width,
-                        uint32_t L, bool tree, bool check_range, pwmhip_ws* ws, const void* h_stream) {
+                        uint32_t L, bool tree, bool check_range, pwmhip_ws* ws, const void* h_stream,
+                        const NoAlloc* na) {
+  if (na && (h_stream || L < 2 || L > 4))
+    throw status_error{PWMHIP_ERR_UNSUPPORTED, "allocation-free builds: 2 to 4 levels, device text"};
   if (h_stream && (L < 2 || L > 4 || small_k1_disabled() || super_disabled()))
     throw status_error{PWMHIP_ERR_UNSUPPORTED, "streamed texts: 2 to 4 levels"};
   if (L > 4 && !small_k1_disabled()) {
@@ -3406,9 +3414,14 @@ void build_plain_device(pwmhip_ctx* ctx, const void* d_text, uint64_t n, uint32_
   const uint32_t wide_nstep = wide_gather_steps();
   const uint64_t wide_wpl = (words_for(n) + 128ull * wide_nstep - 1) / (128ull * wide_nstep);
 
-  // group plan
-  std::vector<uint32_t> groups;
-  for (uint32_t l = 0; l < L; l += kMaxGroup) groups.push_back(std::min<uint32_t>(kMaxGroup, L - l));
+  // group plan (at most 64 / kMaxGroup groups; a fixed array, no heap)
+  struct {
+    uint32_t v[PWMHIP_MAX_LEVELS / kMaxGroup];
+    uint32_t n = 0;
+    size_t size() const { return n; }
+    uint32_t operator[](size_t i) const { return v[i]; }
+  } groups;
+  for (uint32_t l = 0; l < L; l += kMaxGroup) groups.v[groups.n++] = std::min<uint32_t>(kMaxGroup, L - l);
 
   // scratch sizing
   const uint32_t gmax = groups[0];
@@ -3456,11 +3469,20 @@ void build_plain_device(pwmhip_ctx* ctx, const void* d_text, uint64_t n, uint32_
     sizing.cap = ~size_t(0) >> 1;
     carve(sizing, temp_levels, temp_payload, S_ping, S_pong, counts, pref, ztot, totals, gstart, zeros_dev,
           sym_or, tiles, nepref, netot, nebase, level_ne, tab_dst, tab_src);
-    ctx->scratch.reset();
-    ctx->scratch.ensure(sizing.used);
+    if (na) {
+      if (sizing.used > na->cap) throw status_error{PWMHIP_ERR_BUDGET, "text too long for the static scratch"};
+    } else {
+      ctx->scratch.reset();
+      ctx->scratch.ensure(sizing.used);
+    }
   }
-  carve(ctx->scratch, temp_levels, temp_payload, S_ping, S_pong, counts, pref, ztot, totals, gstart, zeros_dev,
-        sym_or, tiles, nepref, netot, nebase, level_ne, tab_dst, tab_src);
+  arena na_view;  // the static scratch, carved but never allocated or released
+  if (na) {
+    na_view.base = na->scratch;
+    na_view.cap = na->cap;
+  }
+  carve(na ? na_view : ctx->scratch, temp_levels, temp_payload, S_ping, S_pong, counts, pref, ztot, totals, gstart,
+        zeros_dev, sym_or, tiles, nepref, netot, nebase, level_ne, tab_dst, tab_src);
   PWM_HIP_TRY(hipMemsetAsync(sym_or, 0, 8, s));
   PWM_HIP_TRY(hipMemsetAsync(zeros_dev, 0, 64 * 8, s));
 
@@ -3495,7 +3517,7 @@ void build_plain_device(pwmhip_ctx* ctx, const void* d_text, uint64_t n, uint32_
     k1.zeros_out = reinterpret_cast<unsigned long long*>(zeros_dev + l0);
     k1.sym_or = sym_or;
     k1.nchunks = nchunks;
-    k1.lut = compress_table(ctx);
+    k1.lut = na ? na->lut : compress_table(ctx);
     if (h_stream) {
       stream_local_build(ctx, h_stream, n, width, L, k1);
     } else if (sup) {
@@ -3605,15 +3627,17 @@ void build_plain_device(pwmhip_ctx* ctx, const void* d_text, uint64_t n, uint32_
   }
   PWM_HIP_TRY(hipEventRecord(ctx->ev[3], s));
 
-  // results: zeros per level and the symbol OR
-  if (ctx->pinned_words < 65) {
+  // results: zeros per level and the symbol OR (the allocation-free build
+  // reads them back into host stack memory instead of the pinned buffer)
+  uint64_t na_res[65] = {};
+  if (!na && ctx->pinned_words < 65) {
     if (ctx->pinned) (void)hipHostFree(ctx->pinned);
     void* hp = nullptr;
     PWM_HIP_TRY(hipHostMalloc(&hp, 128 * 8, hipHostMallocDefault));
     ctx->pinned = static_cast<uint64_t*>(hp);
     ctx->pinned_words = 128;
   }
-  uint64_t* res = ctx->pinned;
+  uint64_t* res = na ? na_res : ctx->pinned;
   PWM_HIP_TRY(hipMemcpyAsync(res, zeros_dev, L * 8, hipMemcpyDeviceToHost, s));
   PWM_HIP_TRY(hipMemcpyAsync(res + 64, sym_or, 8, hipMemcpyDeviceToHost, s));
   PWM_HIP_TRY(hipStreamSynchronize(s));
@@ -4000,3 +4024,77 @@ void build_batch_device(pwmhip_ctx* ctx, const void* const* d_texts, const uint6
 
 }  // namespace pwm
 
+// ---------------------------------------------------------------------------
+// The allocation-free build (pwmhip_build_noalloc): plain builds of 2 to 4
+// levels from a device text into caller-provided output memory, with every
+// temporary in one static device array.  Layout of the array: the 64 KiB
+// compress table, then the build's temporaries (up to kNoAllocTempBytes).
+// Each device that loads the library holds the array until the process ends.
+// ---------------------------------------------------------------------------
+namespace {
+constexpr size_t kNoAllocLutBytes = size_t(1) << 16;
+constexpr size_t kNoAllocTempBytes = size_t(128) << 20;
+constexpr int kNoAllocMaxDevices = 64;
+}  // namespace
+
+static __device__ __align__(256) uint8_t scratch[kNoAllocLutBytes + kNoAllocTempBytes];
+
+namespace pwm {
+namespace {
+std::mutex g_noalloc_mu;
+void* g_noalloc_ptr[kNoAllocMaxDevices] = {};  // device address of `scratch`, per device
+}  // namespace
+
+void noalloc_setup(pwmhip_ctx* ctx) {
+  if (ctx->device < 0 || ctx->device >= kNoAllocMaxDevices) return;  // (the entry point reports it)
+  std::lock_guard<std::mutex> g(g_noalloc_mu);
+  if (g_noalloc_ptr[ctx->device]) return;
+  // with the context's device current: each device has its own copy of the
+  // array at its own address
+  void* p = nullptr;
+  PWM_HIP_TRY(hipGetSymbolAddress(&p, HIP_SYMBOL(scratch)));
+  hipLaunchKernelGGL(k_make_lut, dim3(256), dim3(256), 0, ctx->stream, static_cast<uint8_t*>(p));
+  PWM_HIP_TRY(hipGetLastError());
+  PWM_HIP_TRY(hipStreamSynchronize(ctx->stream));
+  g_noalloc_ptr[ctx->device] = p;
+}
+
+}  // namespace pwm
+
+extern "C" int pwmhip_build_noalloc(pwmhip_ctx* ctx, const void* d_text, uint64_t size, uint32_t word_width,
+                                    uint32_t levels, uint32_t algo, uint32_t flags, uint64_t* d_out,
+                                    uint64_t out_words, pwmhip_ws* out) {
+  if (!ctx || !out) return PWMHIP_ERR_INVALID;
+  if (word_width != 1 && word_width != 2 && word_width != 4 && word_width != 8) return PWMHIP_ERR_INVALID;
+  if (algo > PWMHIP_ALGO_NAIVE) return PWMHIP_ERR_UNSUPPORTED;  // plain shapes only
+  if (levels < 2 || levels > 4 || levels > 8 * word_width) return PWMHIP_ERR_UNSUPPORTED;
+  if (flags & ~uint32_t(PWMHIP_FLAG_TREE | PWMHIP_FLAG_CHECK_RANGE | PWMHIP_FLAG_TEXT_ON_DEVICE))
+    return PWMHIP_ERR_INVALID;
+  if (size && (!d_text || !d_out)) return PWMHIP_ERR_INVALID;
+  const bool tree = (flags & PWMHIP_FLAG_TREE) != 0;
+  pwmhip_ws ws{};
+  ws.is_tree = tree;
+  ws.text_size = size;
+  if (size == 0) {  // wx_ppc.hpp:39-45: empty structure
+    *out = ws;
+    return PWMHIP_OK;
+  }
+  const uint64_t bits[4] = {size, size, size, size};
+  // the layout of pwmhip_build's output, in the caller's memory (not owned:
+  // alloc_bytes stays 0, pwmhip_ws_free frees nothing of it)
+  if (pwm::layout_structure(&ws, levels, bits) > out_words) return PWMHIP_ERR_INVALID;
+  ws.d_words = d_out;
+  const int st = pwm::guarded([&] {
+    PWM_HIP_TRY(hipSetDevice(ctx->device));
+    void* base = ctx->device < kNoAllocMaxDevices ? pwm::g_noalloc_ptr[ctx->device] : nullptr;
+    if (!base) throw pwm::status_error{PWMHIP_ERR_UNSUPPORTED, "no static scratch on this device"};
+    const pwm::NoAlloc na{static_cast<uint8_t*>(base) + kNoAllocLutBytes, kNoAllocTempBytes,
+                          static_cast<const uint8_t*>(base)};
+    pwm::build_plain_device(ctx, d_text, size, word_width, levels, tree, (flags & PWMHIP_FLAG_CHECK_RANGE) != 0,
+                            &ws, nullptr, &na);
+    if (!tree && pwm::zeros_last_is_zero(algo)) ws.zeros[levels - 1] = 0;
+  });
+  if (st != PWMHIP_OK) return st;
+  *out = ws;
+  return PWMHIP_OK;
+}

@@ -87,6 +87,9 @@ int pwmhip_create(pwmhip_ctx** out, int device) {
     ctx->num_cus = cus > 0 ? cus : 256;
     PWM_HIP_TRY(hipStreamCreateWithFlags(&ctx->stream, hipStreamNonBlocking));
     for (auto& e : ctx->ev) PWM_HIP_TRY(hipEventCreate(&e));
+    // the allocation-free build's static scratch on this device; a failure
+    // here only disables that entry point (it then reports UNSUPPORTED)
+    (void)pwm::guarded([&] { pwm::noalloc_setup(ctx); });
   });
   if (st != PWMHIP_OK) {
     pwmhip_destroy(ctx);

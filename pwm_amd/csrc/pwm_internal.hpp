@@ -159,9 +159,22 @@ const uint8_t* compress_table(pwmhip_ctx* ctx);
 // ws->zeros with the true zero counts (matrix).  Throws on error.
 // h_stream: a host text copied to the device in slices while the local
 // build runs (d_text unused); only 2..4 levels (ERR_UNSUPPORTED otherwise).
+// no_alloc: the allocation-free variant's static scratch (temporaries), its
+// compress table and its capacity; the build then allocates nothing (device
+// or pinned host memory) and fails with PWMHIP_ERR_BUDGET before any work if
+// its temporaries do not fit.
+struct NoAlloc {
+  void* scratch;
+  size_t cap;
+  const uint8_t* lut;
+};
 void build_plain_device(pwmhip_ctx* ctx, const void* d_text, uint64_t n, uint32_t width,
                         uint32_t levels, bool tree, bool check_range, pwmhip_ws* ws,
-                        const void* h_stream = nullptr);
+                        const void* h_stream = nullptr, const NoAlloc* no_alloc = nullptr);
+
+// Sets up the allocation-free build on the context's device (once per
+// device): the static scratch's device address and its compress table.
+void noalloc_setup(pwmhip_ctx* ctx);
 
 // zeros[levels-1] is never computed by these reference algorithms
 // (ppc.hpp:19-90, ppc_ss.hpp:19-70, pc_ss.hpp:13-33, wx_dd_pc_ss.hpp).
@@ -186,6 +199,9 @@ void build_batch_device(pwmhip_ctx* ctx, const void* const* d_texts, const uint6
 // and freed otherwise.
 void alloc_structure(pwmhip_ws* ws, uint64_t levels, const uint64_t* bits,
                      uint64_t* reuse = nullptr, uint64_t reuse_bytes = 0);
+// The same layout without the allocation: fills levels, offsets, level bits
+// and returns the words it needs.
+uint64_t layout_structure(pwmhip_ws* ws, uint64_t levels, const uint64_t* bits);
 
 // Exclusive scan over chunks of counts[c*nslot + slot] (one block per slot).
 // tile_scratch holds scan_scratch_words(nchunks, nslot) words.
