@@ -913,7 +913,7 @@ __global__ __launch_bounds__(64 * 16) void k_local_small(K1Params p) {
           T += tot[r];
           E += ex[r];
         }
-        if (j < G) {
+        if (j < G && T) {  // (T: the wave's symbols of the bucket, wave-uniform)
           uint64_t mq = vm;
 #pragma unroll
           for (int i = 0; i < j; ++i) mq &= ((q >> i) & 1) ? P0[i] : ~P0[i];
@@ -1138,16 +1138,19 @@ __global__ __launch_bounds__(64 * WAVES) void k_local_super(K1Params p) {
       uint32_t start = 0;
 #pragma unroll
       for (int q = 0; q < nb; ++q) {
-        uint32_t T = 0, E = 0;
+        uint32_t T = 0, E = 0, Wq = 0;
 #pragma unroll
         for (int r = q; r < NQ; r += nb) {
           T += stot[r];
           E += spre[r] + ex[r];
+          Wq += tot[r];  // this wave's symbols of the bucket (wave-uniform)
         }
-        uint64_t mq = vm;
+        if (Wq) {  // a bucket the wave does not hold has no bits to place
+          uint64_t mq = vm;
 #pragma unroll
-        for (int i = 0; i < j; ++i) mq &= ((q >> i) & 1) ? P0[i] : ~P0[i];
-        or_bits(img[j - 1], start + E, compress64h(P0[j], mq, lut), __popcll(mq));
+          for (int i = 0; i < j; ++i) mq &= ((q >> i) & 1) ? P0[i] : ~P0[i];
+          or_bits(img[j - 1], start + E, compress64h(P0[j], mq, lut), __popcll(mq));
+        }
         cslot[nb - 2 + q] = T;
         start += T;
       }
@@ -1373,16 +1376,19 @@ __global__ __launch_bounds__(64 * WAVES) void k_local_super_batch(K1Params p, co
       uint32_t start = 0;
 #pragma unroll
       for (int q = 0; q < nb; ++q) {
-        uint32_t T = 0, E = 0;
+        uint32_t T = 0, E = 0, Wq = 0;
 #pragma unroll
         for (int r = q; r < NQ; r += nb) {
           T += stot[r];
           E += spre[r] + ex[r];
+          Wq += tot[r];  // this wave's symbols of the bucket (wave-uniform)
         }
-        uint64_t mq = vm;
+        if (Wq) {  // a bucket the wave does not hold has no bits to place
+          uint64_t mq = vm;
 #pragma unroll
-        for (int i = 0; i < j; ++i) mq &= ((q >> i) & 1) ? P0[i] : ~P0[i];
-        or_bits(img[j - 1], start + E, compress64h(P0[j], mq, lut), __popcll(mq));
+          for (int i = 0; i < j; ++i) mq &= ((q >> i) & 1) ? P0[i] : ~P0[i];
+          or_bits(img[j - 1], start + E, compress64h(P0[j], mq, lut), __popcll(mq));
+        }
         cslot[nb - 2 + q] = T;
         start += T;
       }
@@ -1566,18 +1572,23 @@ __global__ __launch_bounds__(64 * kSuper) void k_huff_super(HuffSuperParams hp, 
         const uint32_t rk = hp.rank[j][q];
         if (rk == 0xffu) continue;  // no node
         used[j - 1] += T[q];
-        uint32_t start = 0, E = 0;
+        uint32_t start = 0, E = 0, Wq = 0;
 #pragma unroll
         for (int q2 = 0; q2 < nb; ++q2) {
           const uint32_t r2 = hp.rank[j][q2];
           if (r2 != 0xffu && r2 < rk) start += T[q2];
         }
 #pragma unroll
-        for (int r = q; r < NQ; r += nb) E += spre[r] + ex[r];
-        uint64_t mq = vm;
+        for (int r = q; r < NQ; r += nb) {
+          E += spre[r] + ex[r];
+          Wq += tot[r];  // this wave's symbols of the node (wave-uniform)
+        }
+        if (Wq) {
+          uint64_t mq = vm;
 #pragma unroll
-        for (int i = 0; i < j; ++i) mq &= ((q >> i) & 1) ? P0[i] : ~P0[i];
-        or_bits(img[j - 1], start + E, compress64h(P0[j], mq, lut), __popcll(mq));
+          for (int i = 0; i < j; ++i) mq &= ((q >> i) & 1) ? P0[i] : ~P0[i];
+          or_bits(img[j - 1], start + E, compress64h(P0[j], mq, lut), __popcll(mq));
+        }
         crow = lane == int(hp.slot0[j] + rk) ? T[q] : crow;
       }
     }
