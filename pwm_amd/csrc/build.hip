@@ -2557,6 +2557,11 @@ struct FusedRuns {
   uint64_t units, ntiles, csyms, wave_bits, wpl;
   uint32_t nslot;
   int tree;
+  // slots from sc_slot0 on are the next pass's order (a reorder follows):
+  // their (unit, bucket) positions go to dstpos[u * (nslot - sc_slot0) + b]
+  // instead of run tables; sc_slot0 == nslot: none
+  uint64_t* dstpos;
+  uint32_t sc_slot0;
 };
 
 __global__ __launch_bounds__(1024) void k_runs_fused(FusedRuns p) {
@@ -2627,6 +2632,10 @@ __global__ __launch_bounds__(1024) void k_runs_fused(FusedRuns p) {
       oz += wsum[1][w];
     }
   const uint64_t d = ov + iv - x, e = oz + iz - 1;  // destination start, entry in the level
+  if (slot >= p.sc_slot0) {
+    p.dstpos[c * (p.nslot - p.sc_slot0) + (slot - p.sc_slot0)] = d;
+    return;
+  }
   uint64_t loc = 0;
   for (uint32_t bb = s0; bb < slot; ++bb) loc += p.counts[c * p.nslot + bb];
   p.tab_dst[uint64_t(s0) * p.units + e] = d;
@@ -3248,14 +3257,39 @@ void build_passes_device(pwmhip_ctx* ctx, const void* d_text, uint64_t n, uint32
       const uint64_t ntiles = (nu + kTile - 1) / kTile;
       hipLaunchKernelGGL(k_tile_sums2, dim3(uint32_t(ntiles), nslot), dim3(256), 0, s, counts, U64(3), U64(2),
                          nu, nslot, ntiles);
-      hipLaunchKernelGGL(k_tile_scan, dim3(nslot), dim3(1024), 0, s, U64(3), U64(4), ntiles);
-      hipLaunchKernelGGL(k_tile_scan, dim3(nslot), dim3(1024), 0, s, U64(2), U64(5), ntiles);
-      const uint32_t jb = last ? g - 1 : g;  // sub-levels with bucket starts
-      hipLaunchKernelGGL(k_bucket_starts, dim3(jb), dim3(256), 0, s, U64(4), U64(6), 0, (uint64_t*)nullptr);
-      hipLaunchKernelGGL(k_bucket_starts, dim3(jb), dim3(256), 0, s, U64(5), U64(7), 0, U64(8));
-      hipLaunchKernelGGL(k_apply_runs, dim3(uint32_t(ntiles), nslot), dim3(1024), 0, s, counts, U64(3), U64(2),
-                         U64(6), U64(7), U64(9), U64(10), nu, nslot, ntiles, sup ? U64(16) : U64(11),
-                         last ? nslot : (1u << g) - 2u, usyms);
+      // every block scans the tile sums it needs: bucket starts, run tables,
+      // entry counts, the next order's positions and the wide stitch's wave
+      // starts from one launch (each block reads all tiles of its level)
+      const bool fused = ntiles <= 1024;
+      if (fused) {
+        FusedRuns fr{};
+        fr.counts = counts;
+        fr.tsum = U64(3);
+        fr.tnz = U64(2);
+        fr.tab_dst = U64(9);
+        fr.tab_src = U64(10);
+        fr.level_ne = U64(8);
+        fr.wave_first = sup ? static_cast<uint32_t*>(b[18]) : nullptr;
+        fr.units = nu;
+        fr.ntiles = ntiles;
+        fr.csyms = usyms;
+        fr.wave_bits = 128ull * 64 * wide_nstep;
+        fr.wpl = wide_wpl;
+        fr.nslot = nslot;
+        fr.tree = 0;
+        fr.dstpos = sup ? U64(16) : U64(11);
+        fr.sc_slot0 = last ? nslot : (1u << g) - 2u;
+        hipLaunchKernelGGL(k_runs_fused, dim3(uint32_t(ntiles), nslot), dim3(1024), 0, s, fr);
+      } else {
+        hipLaunchKernelGGL(k_tile_scan, dim3(nslot), dim3(1024), 0, s, U64(3), U64(4), ntiles);
+        hipLaunchKernelGGL(k_tile_scan, dim3(nslot), dim3(1024), 0, s, U64(2), U64(5), ntiles);
+        const uint32_t jb = last ? g - 1 : g;  // sub-levels with bucket starts
+        hipLaunchKernelGGL(k_bucket_starts, dim3(jb), dim3(256), 0, s, U64(4), U64(6), 0, (uint64_t*)nullptr);
+        hipLaunchKernelGGL(k_bucket_starts, dim3(jb), dim3(256), 0, s, U64(5), U64(7), 0, U64(8));
+        hipLaunchKernelGGL(k_apply_runs, dim3(uint32_t(ntiles), nslot), dim3(1024), 0, s, counts, U64(3), U64(2),
+                           U64(6), U64(7), U64(9), U64(10), nu, nslot, ntiles, sup ? U64(16) : U64(11),
+                           last ? nslot : (1u << g) - 2u, usyms);
+      }
       if (sup && !last)
         hipLaunchKernelGGL(k_chunk_dstpos, dim3(uint32_t(units)), dim3(256), 0, s, U64(16),
                            static_cast<const uint32_t*>(b[17]), U64(11), nchunks, 1u << g, pw);
@@ -3272,7 +3306,8 @@ void build_passes_device(pwmhip_ctx* ctx, const void* d_text, uint64_t n, uint32
         wp.n = n;
         wp.wpl = wide_wpl;
         wp.nstep = wide_nstep;
-        hipLaunchKernelGGL(k_wave_starts, dim3(uint32_t((wide_wpl + 255) / 256), g - 1), dim3(256), 0, s, wp);
+        if (!fused)
+          hipLaunchKernelGGL(k_wave_starts, dim3(uint32_t((wide_wpl + 255) / 256), g - 1), dim3(256), 0, s, wp);
         if (i == 0) PWM_HIP_TRY(hipEventRecord(ctx->ev[2], s));
         hipLaunchKernelGGL(k_gather_wide, dim3(uint32_t((wide_wpl + 3) / 4), g - 1), dim3(256), 0, s, wp);
       } else if (i == 0) {
@@ -3565,6 +3600,7 @@ void build_plain_device(pwmhip_ctx* ctx, const void* d_text, uint64_t n, uint32_
         fr.wpl = wide_wpl;
         fr.nslot = nslot;
         fr.tree = tree_direct ? 1 : 0;
+        fr.sc_slot0 = nslot;
         hipLaunchKernelGGL(k_runs_fused, dim3(uint32_t(ntiles), nslot), dim3(1024), 0, s, fr);
       } else {
         hipLaunchKernelGGL(k_tile_scan, dim3(nslot), dim3(1024), 0, s, tsum, totals, ntiles);
