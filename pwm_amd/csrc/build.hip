@@ -458,6 +458,16 @@ __device__ __forceinline__ uint32_t wave_incl_scan(uint32_t v) {
   return v;
 }
 
+// Sums over each row of 16 lanes, in the row's last lane (DPP row shifts,
+// no LDS traffic); other lanes hold partial sums.
+__device__ __forceinline__ uint32_t row_sum16(uint32_t v) {
+  v += __builtin_amdgcn_update_dpp(0u, v, 0x111, 0xf, 0xf, false);  // row_shr:1
+  v += __builtin_amdgcn_update_dpp(0u, v, 0x112, 0xf, 0xf, false);  // row_shr:2
+  v += __builtin_amdgcn_update_dpp(0u, v, 0x114, 0xf, 0xf, false);  // row_shr:4
+  v += __builtin_amdgcn_update_dpp(0u, v, 0x118, 0xf, 0xf, false);  // row_shr:8
+  return v;
+}
+
 // Orders this wave's LDS accesses (lanes exchange data through LDS).
 __device__ __forceinline__ void wave_sync() {
   __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront", "local");
@@ -742,6 +752,16 @@ __device__ __forceinline__ uint64_t compress64h(uint64_t x, uint64_t m, const ui
   return uint64_t(lo) | (uint64_t(hi) << __builtin_popcount(uint32_t(m)));
 }
 
+// A lane's bits of plane x under bucket mask mq, packed.  When all of the
+// wave's symbols fall in the bucket (in_bucket == wave_syms, both
+// wave-uniform) every lane's mask is its valid-symbol mask, a prefix of the
+// word, and x holds no bits past it: the bits are x as it is, and the wave
+// skips the table compress.
+__device__ __forceinline__ uint64_t bucket_bits(uint64_t x, uint64_t mq, uint32_t in_bucket, uint32_t wave_syms,
+                                                const uint8_t* lut) {
+  return in_bucket == wave_syms ? x : compress64h(x, mq, lut);
+}
+
 // With SC (a pass followed by a symbol reorder) the bucket counts of the
 // order after all G levels (2^G buckets, the next pass's level order) are
 // produced as well, as "sub-level G".
@@ -918,7 +938,7 @@ __global__ __launch_bounds__(64 * 16) void k_local_small(K1Params p) {
 #pragma unroll
           for (int i = 0; i < j; ++i) mq &= ((q >> i) & 1) ? P0[i] : ~P0[i];
           const uint32_t len = __popcll(mq);
-          or_bits(buf[j - 1], start + E, compress64h(P0[j], mq, lut), len);
+          or_bits(buf[j - 1], start + E, bucket_bits(P0[j], mq, T, Ec, lut), len);
         }
         cslot[nb - 2 + q] = T;
         start += T;
@@ -1149,7 +1169,7 @@ __global__ __launch_bounds__(64 * WAVES) void k_local_super(K1Params p) {
           uint64_t mq = vm;
 #pragma unroll
           for (int i = 0; i < j; ++i) mq &= ((q >> i) & 1) ? P0[i] : ~P0[i];
-          or_bits(img[j - 1], start + E, compress64h(P0[j], mq, lut), __popcll(mq));
+          or_bits(img[j - 1], start + E, bucket_bits(P0[j], mq, Wq, Ec, lut), __popcll(mq));
         }
         cslot[nb - 2 + q] = T;
         start += T;
@@ -1182,6 +1202,381 @@ __global__ __launch_bounds__(64 * WAVES) void k_local_super(K1Params p) {
   }
 #pragma unroll
   for (int d = 32; d >= 1; d >>= 1) my_or |= __shfl_xor(my_or, d, 64);
+  if (lane == 0 && my_or) atomicOr(&red[G], (unsigned long long)my_or);
+  __syncthreads();
+  if (threadIdx.x == 0) {
+    for (int j = 0; j < G; ++j)
+      if (red[j]) atomicAdd(p.zeros_out + j, red[j]);
+    if (red[G]) atomicOr(p.sym_or, red[G]);
+  }
+}
+
+// ---- K1 with bucket-0 runs written in place ----------------------------------
+// Bucket 0 of a sub-level (the symbols whose bits of the levels above are all
+// zero) comes first in its level, so a unit's bucket-0 run starts at the sum
+// of the earlier units' bucket-0 counts: none of the other buckets' totals is
+// needed.  Every unit publishes its bucket-0 counts as one word with a
+// "published" bit in each half.  The launch is cooperative (every workgroup
+// resident) and workgroup b takes units b, b + G, b + 2G, ...; its running
+// prefix goes from one of its units to the next by adding the counts of the
+// G - 1 units in between (the gap), which the other workgroups publish at
+// about the same time.  Waves 0..NSUB-1 keep one sub-level's prefix each:
+// a unit's gap entries are loaded one step after it was built, right after
+// the text prefetch, and summed at the next step's start, where the step
+// waits for every load anyway; the unit's in-place words go out in that
+// step, so every unit's image stays in LDS for three steps (three buffers).
+// The words that lie wholly inside a unit's bucket-0 run go from the image
+// straight to the level, shifted to the run's bit offset; the stitch builds
+// every other word (the other buckets' runs and the words a run shares with
+// its neighbours) from the image words written to scratch: word 0 and every
+// word from the run's last two on.
+struct DirectParams {
+  unsigned long long* agg;    // [unit]: bucket-0 count of sub-level j in the low 31
+                              // bits of dword j - 1, bit 31 of both dwords = published
+  unsigned long long* fault;  // nonzero: a wait for a unit's counts gave up
+  uint64_t* dst[kMaxGroup];   // final sub-level j at dst[j]
+};
+
+constexpr uint32_t kAggField = 32;
+constexpr unsigned long long kAggFlag = 0x8000000080000000ull;  // in both dwords: a dword read alone is whole
+constexpr uint32_t kGapEntries = 256;      // gap entries loaded a step ahead (grid <= 257)
+constexpr uint32_t kSpinLimit = 1u << 16;  // reloads of one entry before its wait gives up
+
+__device__ __forceinline__ unsigned long long agg_load(const unsigned long long* p) {
+  return __hip_atomic_load(p, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+}
+
+__device__ __forceinline__ uint64_t wave_sum64(uint64_t v) {
+#pragma unroll
+  for (int d = 32; d >= 1; d >>= 1) v += __shfl_xor(v, d, 64);
+  return v;
+}
+
+template <int G>
+__global__ __launch_bounds__(64 * kSuper) void k_local_direct(K1Params p, DirectParams d) {
+  static_assert(G >= 2 && G <= 3, "two or three levels (three LDS images of each sub-level)");
+  constexpr int WAVES = kSuper;
+  constexpr int NSUB = G - 1;      // sub-levels 1..G-1
+  constexpr int JC = G - 1;        // last sub-level
+  constexpr int NQ = 1 << JC;      // its buckets
+  constexpr int NF = NQ + 1;       // bucket counts + the last level's zeros
+  constexpr int NS = (NF + 1) / 2; // packed scans
+  constexpr uint32_t kFieldMask = 0x7fffffffu;
+  __shared__ __align__(16) uint8_t lut[65536];
+  __shared__ uint64_t img[3][NSUB][kSuperWords];
+  __shared__ uint32_t wt[WAVES][NQ];
+  __shared__ unsigned long long red[G + 1];  // block sums: zeros per level, symbol OR
+  __shared__ uint32_t part[WAVES][NSUB];     // per wave: its share of a gap's bucket-0 counts
+  const int lane = threadIdx.x & 63;
+  const int wid = threadIdx.x >> 6;
+  const int uwid = __builtin_amdgcn_readfirstlane(wid);
+  const uint64_t nsuper = (p.nchunks + WAVES - 1) / WAVES;
+  const uint64_t G_ = gridDim.x;
+  if (threadIdx.x <= G) red[threadIdx.x] = 0;
+  for (uint32_t i = threadIdx.x; i < 3 * NSUB * kSuperWords; i += 64 * WAVES) (&img[0][0][0])[i] = 0;
+  load_lut<WAVES>(lut, p.lut);  // (ends with a barrier)
+
+  uint32_t zsum[G];
+#pragma unroll
+  for (int j = 0; j < G; ++j) zsum[j] = 0;
+  uint64_t my_or = 0;
+  const bool fast_mode = (reinterpret_cast<uintptr_t>(p.text) & 15) == 0;
+  auto full_chunk = [&](uint64_t c) { return fast_mode && c < p.nchunks && (c + 1) * kChunkSyms <= p.n; };
+  auto load_chunk = [&](uint64_t c, uint4* dd) {
+    const u32x4_t* src = reinterpret_cast<const u32x4_t*>(static_cast<const uint8_t*>(p.text) + c * kChunkSyms) +
+                         4 * lane;
+#pragma unroll
+    for (int i = 0; i < 4; ++i) {
+      const u32x4_t v = ld_stream(src + i);
+      dd[i] = make_uint4(v.x, v.y, v.z, v.w);
+    }
+  };
+  // words wholly inside the bucket-0 runs [D[j], D[j] + cl[j]) of sub-levels
+  // j + 1, from LDS image b, one per thread
+  auto write_in_place = [&](int b, const uint64_t* D, const uint32_t* cl) {
+#pragma unroll
+    for (int j = 0; j < NSUB; ++j) {
+      const uint64_t wf = (D[j] + 63) >> 6, wl = (D[j] + cl[j]) >> 6;
+      const uint64_t w0 = D[j] >> 6;
+      const uint32_t s = uint32_t(D[j] & 63);
+      const uint64_t w = wf + threadIdx.x;  // (a run holds at most 1024 whole words)
+      if (w < wl) {
+        const uint32_t m = uint32_t(w - w0);  // 1 <= m when s > 0, m <= 1023
+        uint64_t v = img[b][j][m];
+        if (s) v = (v << s) | (img[b][j][m - 1] >> (64 - s));
+        st_stream(d.dst[j + 1] + w, brev64(v));
+      }
+    }
+  };
+
+  // Running prefix, the same in every wave: incl[j] = sub-level j + 1's
+  // bucket-0 count of every unit up to the last one resolved; c0a / c0b =
+  // the counts of the units built one and two steps ago.  The gap of unit u
+  // is [u - G + 1, u) (from 0 for the first round); lane l < 16 NSUB of wave
+  // w holds its entry 16 w + (l & 15) of sub-level (l >> 4) + 1 (G <= 257).
+  uint64_t incl[NSUB];
+  uint32_t c0a[NSUB], c0b[NSUB];
+#pragma unroll
+  for (int j = 0; j < NSUB; ++j) incl[j] = c0a[j] = c0b[j] = 0;
+  // (values touched once a step live in vector registers: the scalar file
+  // is full, and scalar spills cost vector instructions every step)
+  auto in_vgpr = [](auto& x) { asm volatile("" : "+v"(x)); };
+  // (unit indices fit 32 bits: the count array has one entry per unit)
+  const uint32_t G32 = uint32_t(G_);
+  const uint32_t gi = 16 * uint32_t(uwid) + uint32_t(lane & 15);
+  const uint32_t gj = uint32_t(lane) >> 4;
+  const bool glane = gj < uint32_t(NSUB);
+  const uint32_t* const agg_l = reinterpret_cast<const uint32_t*>(d.agg) + 2 * gi + (glane ? gj : 0);
+  uint32_t gvv = 0;  // the entry loaded a step ago
+  bool gave_up = false;
+  auto gap_lo = [&](uint32_t u) { return u >= G32 ? u - G32 + 1 : 0u; };
+  auto issue_gap = [&](uint32_t u) {  // after the prefetch: waited for at the next step's start
+    const uint32_t lo = gap_lo(u);
+    const bool v = glane && lo + gi < u;  // (otherwise entry lo, which exists, is loaded and not used)
+    gvv = __hip_atomic_load(agg_l + 2 * lo - (v ? 0 : 2 * gi), __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+  };
+  // this wave's share of unit u's gap (the loads have landed) into part[]
+  auto share = [&](uint32_t u) {
+    const uint32_t lo = gap_lo(u);
+    const bool v = glane && lo + gi < u;
+    uint32_t x = v ? gvv : 0x80000000u;
+    if (!(x & 0x80000000u)) {  // not published yet: wait for it
+      uint32_t spins = 0;
+#pragma clang loop unroll(disable)
+      while (!(x & 0x80000000u)) {
+        if (gave_up || ++spins > kSpinLimit) {
+          gave_up = true;
+          atomicOr(d.fault, 1ull);
+          break;
+        }
+        __builtin_amdgcn_s_sleep(2);
+        x = __hip_atomic_load(agg_l + 2 * lo, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+      }
+    }
+    x &= kFieldMask;
+    x = row_sum16(x);  // lanes 15 and 31: the two sub-levels' shares
+    // (read out in uniform control flow: inside a branch taken by lane 0
+    // alone, the compiler may finish the sum for the active lane only)
+    uint32_t sh[NSUB];
+#pragma unroll
+    for (int j = 0; j < NSUB; ++j) sh[j] = __builtin_amdgcn_readlane(x, 16 * j + 15);
+    if (lane == 0) {
+#pragma unroll
+      for (int j = 0; j < NSUB; ++j) part[uwid][j] = sh[j];
+    }
+  };
+  // after a barrier: the run starts of the unit whose shares are in part[]
+  auto finish = [&](uint64_t* D, const uint32_t* cl) {
+    // lane l < 16 NSUB reads wave (l & 15)'s share of sub-level (l >> 4) + 1
+    // (each share is below 2^24: 16 entries of at most 2^16)
+    const uint32_t x = row_sum16(glane ? part[lane & 15][gj] : 0u);
+#pragma unroll
+    for (int j = 0; j < NSUB; ++j) {
+      D[j] = incl[j] + __builtin_amdgcn_readlane(x, 16 * j + 15);
+      incl[j] = D[j] + cl[j];
+    }
+  };
+
+  uint4 pf[4];
+  uint64_t cur = blockIdx.x;  // units blockIdx.x + k * gridDim.x (every workgroup resident)
+  if (cur < nsuper && full_chunk(cur * WAVES + uwid)) load_chunk(cur * WAVES + uwid, pf);
+
+  uint32_t step = 0;
+  for (; cur < nsuper; ++step, cur += G_) {
+    const int buf = int(step % 3), dbuf = int((step + 1) % 3);
+    const uint64_t sc = cur;
+    const uint64_t nxt = sc + G_;
+    const uint64_t c = sc * WAVES + uwid;  // (wave-uniform: chunk addresses stay scalar)
+    const uint64_t cn = nxt * WAVES + uwid;
+    const bool have = c < p.nchunks;
+    const uint32_t Ec = have ? uint32_t(min<uint64_t>(kChunkSyms, p.n - c * kChunkSyms)) : 0u;
+    const uint32_t nw = (Ec + 63) >> 6;
+    const int cnt = max(0, min(64, int(Ec) - 64 * lane));
+
+    uint64_t P0[G];
+#pragma unroll
+    for (int k = 0; k < G; ++k) P0[k] = 0;
+    const bool fc = full_chunk(c);
+    if (fc) {
+      planes_from_bytes<G>(p, pf, P0, my_or);
+    } else if (cnt > 0) {
+      load_planes_text<G, uint8_t>(p, p.text, c * kChunkSyms + 64ull * lane, cnt, P0, my_or);
+    }
+    // this wave's share of the gap of the unit built two steps ago (its
+    // entries were loaded a step ago, after that step's prefetch: the wait
+    // for them falls where the step waits for its stores before the next
+    // prefetch anyway)
+    if (step >= 2) share(uint32_t(sc - 2 * G_));
+    if (nxt < nsuper && full_chunk(cn)) load_chunk(cn, pf);  // prefetch
+    if (step >= 1) issue_gap(uint32_t(sc - G_));              // the gap of the unit built a step ago
+    const uint64_t vm = cnt >= 64 ? ~0ull : (cnt <= 0 ? 0ull : ((1ull << cnt) - 1));
+    if (lane < int(nw)) st_stream(p.out_level0 + c * kChunkWords + lane, brev64(P0[0]));
+
+    // bucket masks of the last sub-level, M[q] = prod_i (bit i of q ? P_i : ~P_i)
+    uint64_t M[NQ];
+    M[0] = vm;
+#pragma unroll
+    for (int i = 0; i < JC; ++i) {
+#pragma unroll
+      for (int q = (1 << i) - 1; q >= 0; --q) {
+        M[q + (1 << i)] = M[q] & P0[i];
+        M[q] = M[q] & ~P0[i];
+      }
+    }
+    uint32_t f[2 * NS];
+#pragma unroll
+    for (int q = 0; q < NQ; ++q) f[q] = __popcll(M[q]);
+    f[NQ] = __popcll(~P0[G - 1] & vm);
+    if (NF < 2 * NS) f[2 * NS - 1] = 0;
+    uint32_t ex[2 * NS], tot[2 * NS];
+#pragma unroll
+    for (int s2 = 0; s2 < NS; ++s2) {
+      const uint32_t v = f[2 * s2] | (f[2 * s2 + 1] << 16);
+      const uint32_t inc2 = wave_incl_scan(v);
+      const uint32_t t = __builtin_amdgcn_readlane(inc2, 63);
+      const uint32_t e = inc2 - v;
+      ex[2 * s2] = e & 0xffffu;
+      ex[2 * s2 + 1] = e >> 16;
+      tot[2 * s2] = t & 0xffffu;
+      tot[2 * s2 + 1] = t >> 16;
+    }
+#pragma unroll
+    for (int j = 0; j < JC; ++j) {
+      uint32_t z = 0;
+#pragma unroll
+      for (int q = 0; q < NQ; ++q)
+        if (!((q >> j) & 1)) z += tot[q];
+      zsum[j] += z;
+    }
+    zsum[G - 1] += tot[NQ];
+    if (lane < NQ) {
+      uint32_t v = 0;
+#pragma unroll
+      for (int q = 0; q < NQ; ++q) v = lane == q ? tot[q] : v;
+      wt[wid][lane] = v;
+    }
+    lds_sync();  // every wave's totals and gap shares are there; image `buf` was cleared
+
+    // the unit's totals and this wave's prefix, per last-level bucket
+    uint32_t stot[NQ], spre[NQ];
+#pragma unroll
+    for (int q = 0; q < NQ; ++q) {
+      const uint32_t v = lane < WAVES ? wt[lane][q] : 0u;
+      const uint32_t i2 = wave_incl_scan(v);
+      stot[q] = __builtin_amdgcn_readlane(i2, WAVES - 1);
+      spre[q] = __builtin_amdgcn_readlane(i2 - v, uwid);
+    }
+    uint32_t c0[NSUB];  // the unit's bucket-0 count per sub-level
+#pragma unroll
+    for (int j = 1; j <= NSUB; ++j) {
+      uint32_t T = 0;
+#pragma unroll
+      for (int r = 0; r < NQ; r += 1 << j) T += stot[r];
+      c0[j - 1] = T;
+    }
+    if (uwid == 0 && lane == 0) {
+      unsigned long long a = 0;
+#pragma unroll
+      for (int j = 0; j < NSUB; ++j) a |= (unsigned long long)c0[j] << (32 * j);
+      __hip_atomic_store(d.agg + sc, a | kAggFlag, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+    }
+    // the run starts of the unit built two steps ago
+    uint64_t D[NSUB];
+    uint32_t cl[NSUB];
+#pragma unroll
+    for (int j = 0; j < NSUB; ++j) cl[j] = c0b[j];
+    if (step >= 2) finish(D, cl);
+#pragma unroll
+    for (int j = 0; j < NSUB; ++j) {
+      in_vgpr(D[j]);
+      in_vgpr(cl[j]);
+    }
+#pragma unroll
+    for (int j = 0; j < NSUB; ++j) {
+      c0b[j] = c0a[j];
+      c0a[j] = c0[j];
+      in_vgpr(c0a[j]);
+      in_vgpr(c0b[j]);
+      in_vgpr(incl[j]);
+    }
+#pragma unroll
+    for (int j = 0; j < G; ++j) in_vgpr(zsum[j]);
+
+    uint32_t cslot[2 * NQ - 2];
+#pragma unroll
+    for (int j = 1; j <= JC; ++j) {
+      const int nb = 1 << j;
+      uint32_t start = 0;
+#pragma unroll
+      for (int q = 0; q < nb; ++q) {
+        uint32_t T = 0, E = 0, Wq = 0;
+#pragma unroll
+        for (int r = q; r < NQ; r += nb) {
+          T += stot[r];
+          E += spre[r] + ex[r];
+          Wq += tot[r];  // this wave's symbols of the bucket (wave-uniform)
+        }
+        if (Wq) {
+          uint64_t mq = vm;
+#pragma unroll
+          for (int i = 0; i < j; ++i) mq &= ((q >> i) & 1) ? P0[i] : ~P0[i];
+          or_bits(img[buf][j - 1], start + E, bucket_bits(P0[j], mq, Wq, Ec, lut), __popcll(mq));
+        }
+        cslot[nb - 2 + q] = T;
+        start += T;
+      }
+    }
+    // the unit built two steps ago goes out
+    if (step >= 2) write_in_place(dbuf, D, cl);
+    lds_sync();  // image `buf` is complete; image `dbuf` and part[] were read
+
+    // scratch: word 0 and the words from the bucket-0 run's last two on
+    const uint64_t s0 = sc * kSuperSyms;
+    const uint32_t nws = uint32_t((min<uint64_t>(kSuperSyms, p.n - s0) + 63) >> 6);
+#pragma unroll
+    for (int j = 0; j < NSUB; ++j) {
+      const uint32_t lo = (c0[j] >> 6) ? (c0[j] >> 6) - 1 : 0;
+      if (threadIdx.x < nws && (threadIdx.x == 0 || threadIdx.x >= lo))
+        st_stream(p.temp_levels + j * p.temp_stride + sc * kSuperWords + threadIdx.x,
+                  brev64(img[buf][j][threadIdx.x]));
+      img[dbuf][j][threadIdx.x] = 0;  // written in place above; cleared before the next step's first barrier
+    }
+    if (uwid == 0) {
+      uint32_t v = 0;
+#pragma unroll
+      for (int i = 0; i < 2 * NQ - 2; ++i) v = lane == i ? cslot[i] : v;
+      if (lane < 2 * NQ - 2) gmem(p.counts)[sc * p.nslot + lane] = v;
+    }
+  }
+  // the last two units built: the one two steps back (its gap loaded in the
+  // last step), then the last one
+  if (step >= 2) {
+    __builtin_amdgcn_s_waitcnt(0x0f70);
+    share(uint32_t(cur - 2 * G_));
+    lds_sync();
+    uint64_t D[NSUB];
+    finish(D, c0b);
+    write_in_place(int((step + 1) % 3), D, c0b);
+  }
+  if (step >= 1) {
+    const uint32_t u = uint32_t(cur - G_);
+    issue_gap(u);
+    __builtin_amdgcn_s_waitcnt(0x0f70);
+    lds_sync();  // part[] was read
+    share(u);
+    lds_sync();
+    uint64_t D[NSUB];
+    finish(D, c0a);
+    write_in_place(int((step + 2) % 3), D, c0a);
+  }
+
+  if (lane == 0) {
+#pragma unroll
+    for (int j = 0; j < G; ++j)
+      if (zsum[j]) atomicAdd(&red[j], (unsigned long long)zsum[j]);
+  }
+#pragma unroll
+  for (int dd = 32; dd >= 1; dd >>= 1) my_or |= __shfl_xor(my_or, dd, 64);
   if (lane == 0 && my_or) atomicOr(&red[G], (unsigned long long)my_or);
   __syncthreads();
   if (threadIdx.x == 0) {
@@ -1387,7 +1782,7 @@ __global__ __launch_bounds__(64 * WAVES) void k_local_super_batch(K1Params p, co
           uint64_t mq = vm;
 #pragma unroll
           for (int i = 0; i < j; ++i) mq &= ((q >> i) & 1) ? P0[i] : ~P0[i];
-          or_bits(img[j - 1], start + E, compress64h(P0[j], mq, lut), __popcll(mq));
+          or_bits(img[j - 1], start + E, bucket_bits(P0[j], mq, Wq, Ec, lut), __popcll(mq));
         }
         cslot[nb - 2 + q] = T;
         start += T;
@@ -1587,7 +1982,7 @@ __global__ __launch_bounds__(64 * kSuper) void k_huff_super(HuffSuperParams hp, 
           uint64_t mq = vm;
 #pragma unroll
           for (int i = 0; i < j; ++i) mq &= ((q >> i) & 1) ? P0[i] : ~P0[i];
-          or_bits(img[j - 1], start + E, compress64h(P0[j], mq, lut), __popcll(mq));
+          or_bits(img[j - 1], start + E, bucket_bits(P0[j], mq, Wq, Ec, lut), __popcll(mq));
         }
         crow = lane == int(hp.slot0[j] + rk) ? T[q] : crow;
       }
@@ -2007,7 +2402,7 @@ __device__ __forceinline__ uint64_t readlane64(uint64_t v, uint32_t lane) {
 // `pending` lanes own an unfinished word and stay pending when the word
 // continues past the cache.
 __device__ __forceinline__ uint64_t gather_word(const uint64_t* src, uint64_t bd, uint64_t bs, uint64_t ws,
-                                                uint64_t we, bool& pending) {
+                                                uint64_t we, bool& pending, uint64_t T0, bool& full0) {
   // last cached entry with start <= ws (entry 0 if none)
   uint32_t idx = 0;
 #pragma unroll
@@ -2018,6 +2413,8 @@ __device__ __forceinline__ uint64_t gather_word(const uint64_t* src, uint64_t bd
   uint64_t d0 = __shfl(bd, int(idx), 64);
   uint64_t d1 = __shfl(bd, int(idx + 1 < 64 ? idx + 1 : 63), 64);
   uint64_t s0 = __shfl(bs, int(idx), 64);
+  // a word wholly inside one bucket-0 run (d0 < T0) was written in place
+  full0 = pending && idx < 63 && d0 <= ws && d1 >= ws + 64 && d0 < T0;
   uint64_t v = 0;
   bool more = pending;
   while (__ballot(more)) {  // wave-uniform loop: shuffles below see every lane
@@ -2067,7 +2464,7 @@ struct RunCache {
 // pieces of single destination words and ORs them into the wave's LDS copy
 // of the window; work is proportional to the runs, not to runs x words.
 __device__ void gather_window_runs(const uint64_t* src, uint64_t* dst, RunCache& rc, uint64_t w0, uint64_t e0,
-                                   unsigned long long* stage) {
+                                   unsigned long long* stage, uint64_t T0) {
   const uint32_t lane = threadIdx.x & 63;
   const uint64_t n = rc.n, NE = rc.NE;
   const uint64_t sw0 = w0 << 6, sw1 = min<uint64_t>(sw0 + 4096, n);
@@ -2083,10 +2480,15 @@ __device__ void gather_window_runs(const uint64_t* src, uint64_t* dst, RunCache&
   const uint64_t e1 = lo;
   stage[lane] = 0;
   wave_sync();
+  uint64_t fullm = 0;  // bit i: word w0 + i lies wholly inside a bucket-0 run (written in place)
   for (uint64_t eb = e0; eb <= e1; eb += 64) {
     const uint64_t e = eb + lane;
     if (e <= e1) {
       const uint64_t d0 = rc.tab_dst[e], d1 = e + 1 < NE ? rc.tab_dst[e + 1] : n, s0 = rc.tab_src[e];
+      if (d0 < T0) {
+        const uint64_t fa = max((d0 + 63) >> 6, w0), fb = min(d1 >> 6, w0 + 64);
+        if (fa < fb) fullm |= (fb - fa == 64 ? ~0ull : ((1ull << (fb - fa)) - 1)) << (fa - w0);
+      }
       const uint64_t a = max(d0, sw0), b = min(d1, sw1);
       for (uint64_t pos = a; pos < b;) {
         const uint32_t off = uint32_t((pos - sw0) & 63);
@@ -2098,13 +2500,17 @@ __device__ void gather_window_runs(const uint64_t* src, uint64_t* dst, RunCache&
     }
   }
   wave_sync();
-  if (((w0 + lane) << 6) < n) st_stream(dst + w0 + lane, uint64_t(stage[lane]));
+#pragma unroll
+  for (int dd = 32; dd >= 1; dd >>= 1) fullm |= __shfl_xor(fullm, dd, 64);
+  if (((w0 + lane) << 6) < n && !((fullm >> lane) & 1)) st_stream(dst + w0 + lane, uint64_t(stage[lane]));
   rc.base = e1;  // the window's last run may continue into the next window
   rc.load(lane);
 }
 
+// T0: words wholly inside one run that starts below T0 (bucket 0 when the
+// local build wrote those words in place) are left alone.
 __device__ __forceinline__ void gather_window(const uint64_t* src, uint64_t* dst, RunCache& rc, uint64_t w0,
-                                              unsigned long long* stage) {
+                                              unsigned long long* stage, uint64_t T0 = 0) {
   const uint32_t lane = threadIdx.x & 63;
   const uint64_t n = rc.n;
   const uint64_t sw0 = w0 << 6, sw1 = min<uint64_t>(sw0 + 4096, n);
@@ -2120,24 +2526,26 @@ __device__ __forceinline__ void gather_window(const uint64_t* src, uint64_t* dst
     el = __popcll(__ballot(rc.bd < sw1)) - 1;
   }
   if (el == 63) {
-    gather_window_runs(src, dst, rc, w0, rc.base + ef, stage);
+    gather_window_runs(src, dst, rc, w0, rc.base + ef, stage, T0);
     return;
   }
   uint64_t v = 0;
+  bool full0 = false;
   if (el - ef < 4) {
     for (uint32_t e = ef; e <= el; ++e) {
       const uint64_t d0 = readlane64(rc.bd, e), d1 = readlane64(rc.bd, e + 1), s0 = readlane64(rc.bs, e);
       const uint64_t a = max(d0, ws), b = min(d1, we);
-      if (live && a < b) {
+      full0 |= d0 <= ws && d1 >= ws + 64 && d0 < T0;
+      if (live && a < b && !full0) {
         const int64_t sb = int64_t(s0) - int64_t(d0) + int64_t(ws);
         v |= window64(src, sb) & range_mask(uint32_t(a - ws), uint32_t(b - ws));
       }
     }
   } else {
     bool pending = live;  // every run of the window is in the cache
-    v = gather_word(src, rc.bd, rc.bs, ws, we, pending);
+    v = gather_word(src, rc.bd, rc.bs, ws, we, pending, T0, full0);
   }
-  if (live) st_stream(dst + w0 + lane, v);
+  if (live && !full0) st_stream(dst + w0 + lane, v);
 }
 
 // Writes the wave's destination words [W0, W0 + 64*nwin) of `nplanes`
@@ -2241,9 +2649,12 @@ __device__ __forceinline__ void gather_range(const uint64_t* tab_dst, const uint
 // with one source offset for the whole wave; four steps issue their loads
 // together.  Steps that a run boundary crosses go through gather_window
 // (64-word windows, one word per lane).
+// T0 > 0: the words wholly inside bucket-0 runs (runs starting below T0) were
+// written in place by the local build and are skipped.
 __device__ __forceinline__ void gather_range_wide(const uint64_t* tab_dst, const uint64_t* tab_src, uint64_t NE,
                                                   const uint64_t* src, uint64_t* dst, uint64_t n, uint64_t W0,
-                                                  uint32_t nstep, uint64_t first, unsigned long long* stage) {
+                                                  uint32_t nstep, uint64_t first, unsigned long long* stage,
+                                                  uint64_t T0 = 0) {
   constexpr uint32_t kSteps = 4;
   const uint32_t lane = threadIdx.x & 63;
   const uint64_t X = W0 << 6;
@@ -2261,8 +2672,9 @@ __device__ __forceinline__ void gather_range_wide(const uint64_t* tab_dst, const
       rc.load(lane);
       el = __popcll(__ballot(rc.bd < gend)) - 1;
     }
-    // steps inside one cached run (whose end is cached too)
-    uint32_t single = 0;
+    // steps inside one cached run (whose end is cached too); a step wholly
+    // inside a bucket-0 run written in place is done
+    uint32_t single = 0, inplace = 0;
     int64_t dl[kSteps];
 #pragma unroll
     for (int k = 0; k < int(kSteps); ++k) {
@@ -2272,8 +2684,14 @@ __device__ __forceinline__ void gather_range_wide(const uint64_t* tab_dst, const
         const uint32_t a = __popcll(__ballot(rc.bd <= sw0)) - 1;
         const uint32_t b = __popcll(__ballot(rc.bd < sw1)) - 1;
         if (a == b) {
-          single |= 1u << k;
-          dl[k] = int64_t(readlane64(rc.bs, a)) - int64_t(readlane64(rc.bd, a));
+          const uint64_t d0 = readlane64(rc.bd, a);
+          if (d0 < T0) {
+            if (readlane64(rc.bd, a + 1) >= sw0 + 8192) inplace |= 1u << k;
+            // (a bucket-0 run ending inside the step: the level's last, partial word)
+          } else {
+            single |= 1u << k;
+            dl[k] = int64_t(readlane64(rc.bs, a)) - int64_t(d0);
+          }
         }
       }
     }
@@ -2316,11 +2734,11 @@ __device__ __forceinline__ void gather_range_wide(const uint64_t* tab_dst, const
     }
 #pragma unroll 1
     for (uint32_t k = 0; k < kSteps; ++k) {
-      if ((single >> k) & 1) continue;
+      if (((single | inplace) >> k) & 1) continue;
       const uint64_t w0 = gw0 + 128ull * k;
       if ((w0 << 6) >= n) break;
-      gather_window(src, dst, rc, w0, stage);
-      if (((w0 + 64) << 6) < n) gather_window(src, dst, rc, w0 + 64, stage);
+      gather_window(src, dst, rc, w0, stage, T0);
+      if (((w0 + 64) << 6) < n) gather_window(src, dst, rc, w0 + 64, stage, T0);
     }
   }
 }
@@ -2337,6 +2755,8 @@ struct WideParams {
   uint64_t units, n, wpl;    // run units, symbols, waves per level
   uint32_t nstep;
   uint64_t nbits[kMaxGroup]; // bits of sub-level j when its own (0: n)
+  const uint64_t* bucket0;   // or nullptr: [j] = bucket 0's total of sub-level j, whose
+                             // runs' whole words the local build wrote in place
 };
 
 // First table entry of every wave of the wide stitch: the last entry whose
@@ -2360,15 +2780,58 @@ __global__ __launch_bounds__(256) void k_wave_starts(WideParams p) {
   p.first[(j - 1) * p.wpl + k] = uint32_t(lo);
 }
 
+// With bucket-0 runs written in place (p.bucket0), the waves whose words all
+// lie below bucket 0's end have nothing left but the words two runs share:
+// they exit, and k_bucket0_edges builds those words.
+__device__ __forceinline__ uint64_t bucket0_skip_words(const WideParams& p, uint32_t j) {
+  const uint64_t wpw = 128ull * p.nstep;  // words per wave
+  return p.bucket0 ? (p.bucket0[j] >> 6) / wpw * wpw : 0;
+}
+
 __global__ __launch_bounds__(256) void k_gather_wide(WideParams p) {
   __shared__ unsigned long long stage[4][64];
   const uint32_t j = blockIdx.y + 1;
   const uint64_t wave = uint64_t(blockIdx.x) * 4 + (threadIdx.x >> 6);
   if (wave >= p.wpl) return;  // whole waves; no barrier below
+  if ((wave + 1) * 128ull * p.nstep <= bucket0_skip_words(p, j)) return;
   const uint64_t toff = uint64_t((1u << j) - 2) * p.units;
   gather_range_wide(p.tab_dst + toff, p.tab_src + toff, p.level_ne[j], p.src + (j - 1) * p.src_stride, p.dst[j],
                     p.nbits[j] ? p.nbits[j] : p.n, wave * 128 * p.nstep, p.nstep,
-                    p.first[(j - 1) * p.wpl + wave], stage[threadIdx.x >> 6]);
+                    p.first[(j - 1) * p.wpl + wave], stage[threadIdx.x >> 6], p.bucket0 ? p.bucket0[j] : 0);
+}
+
+// The words below the skipped waves' end that bucket-0 runs share (those
+// not wholly inside one run, which the local build wrote in place): one
+// thread per run, which builds the word holding the run's end when the run
+// also holds that word's first bit, from the following runs as far as the
+// word goes.  Every other word there lies inside one run.
+__global__ __launch_bounds__(256) void k_bucket0_edges(WideParams p) {
+  const uint32_t j = blockIdx.y + 1;
+  const uint64_t e = uint64_t(blockIdx.x) * 256 + threadIdx.x;
+  const uint64_t NE = p.level_ne[j];
+  if (e >= NE) return;
+  const uint64_t toff = uint64_t((1u << j) - 2) * p.units;
+  const uint64_t* td = p.tab_dst + toff;
+  const uint64_t* ts = p.tab_src + toff;
+  const uint64_t nb = p.nbits[j] ? p.nbits[j] : p.n;
+  const uint64_t T0 = p.bucket0[j];
+  uint64_t d0 = td[e];
+  if (d0 >= T0) return;
+  uint64_t d1 = e + 1 < NE ? td[e + 1] : nb;
+  if ((d1 & 63) == 0) return;
+  const uint64_t w = (d1 - 1) >> 6, ws = w << 6;
+  if (ws < d0 || w >= bucket0_skip_words(p, j)) return;
+  const uint64_t* src = p.src + (j - 1) * p.src_stride;
+  uint64_t v = 0, ee = e, s0 = ts[e];
+  for (;;) {  // (the word lies below T0: bucket-0 runs only)
+    const uint64_t a = max(d0, ws), b = min(d1, ws + 64);
+    if (a < b) v |= window64(src, int64_t(s0) - int64_t(d0) + int64_t(ws)) & range_mask(uint32_t(a - ws), uint32_t(b - ws));
+    if (d1 >= ws + 64 || ++ee >= NE) break;
+    d0 = d1;
+    d1 = ee + 1 < NE ? td[ee + 1] : nb;
+    s0 = ts[ee];
+  }
+  p.dst[j][w] = v;
 }
 
 __global__ __launch_bounds__(256) void k_gather(GatherParams p) {
@@ -2562,6 +3025,7 @@ struct FusedRuns {
   // instead of run tables; sc_slot0 == nslot: none
   uint64_t* dstpos;
   uint32_t sc_slot0;
+  uint64_t* bucket0;  // or nullptr: bucket 0's total of sub-level j at [j]
 };
 
 __global__ __launch_bounds__(1024) void k_runs_fused(FusedRuns p) {
@@ -2575,7 +3039,7 @@ __global__ __launch_bounds__(1024) void k_runs_fused(FusedRuns p) {
     // totals of the level's buckets before this one (output order) and the
     // sums of this slot's earlier tiles
     const uint32_t pb = p.tree ? rev_bits(b, j) : b;
-    uint64_t gv = 0, gz = 0, lz = 0, pv = 0, pz = 0;
+    uint64_t gv = 0, gz = 0, lz = 0, pv = 0, pz = 0, t0 = 0;
     for (uint32_t bb = 0; bb < nb; ++bb) {
       const uint32_t pos = p.tree ? rev_bits(bb, j) : bb;
       const uint64_t* sv = p.tsum + uint64_t(s0 + bb) * p.ntiles;
@@ -2583,6 +3047,7 @@ __global__ __launch_bounds__(1024) void k_runs_fused(FusedRuns p) {
       for (uint64_t i = lane; i < p.ntiles; i += 64) {
         const uint64_t v = sv[i], z = sz[i];
         lz += z;
+        if (bb == 0) t0 += v;
         if (pos < pb) {
           gv += v;
           gz += z;
@@ -2600,11 +3065,15 @@ __global__ __launch_bounds__(1024) void k_runs_fused(FusedRuns p) {
       lz += __shfl_xor(lz, d, 64);
       pv += __shfl_xor(pv, d, 64);
       pz += __shfl_xor(pz, d, 64);
+      t0 += __shfl_xor(t0, d, 64);
     }
     if (lane == 0) {
       base[0] = gv + pv;
       base[1] = gz + pz;
-      if (tile == 0 && b == 0) p.level_ne[j] = lz;
+      if (tile == 0 && b == 0) {
+        p.level_ne[j] = lz;
+        if (p.bucket0) p.bucket0[j] = t0;
+      }
     }
   }
   const uint64_t c = tile * kTile + t;
@@ -2834,6 +3303,26 @@ bool small_k1_disabled() {
 bool super_disabled() {
   const char* e = std::getenv("PWMHIP_CHUNK_RUNS");  // read at every call
   return e && e[0] == '1';
+}
+
+// PWMHIP_STITCH_ALL=1: single texts of 2 or 3 byte levels leave every word
+// of the sub-levels to the stitch (no bucket-0 runs written in place).
+bool direct_disabled() {
+  const char* e = std::getenv("PWMHIP_STITCH_ALL");  // read at every call
+  return e && e[0] == '1';
+}
+
+// Cooperative: the workgroups wait for each other's counts, so all of them
+// must be resident (one per CU: the kernel's LDS allows no second).
+template <int G>
+void launch_direct(pwmhip_ctx* ctx, const K1Params& prm, const DirectParams& dp) {
+  const uint64_t nsuper = (prm.nchunks + kSuper - 1) / kSuper;
+  const uint64_t grid = std::min<uint64_t>({nsuper, uint64_t(ctx->num_cus), uint64_t(kGapEntries) + 1});
+  K1Params a0 = prm;
+  DirectParams a1 = dp;
+  void* args[] = {&a0, &a1};
+  PWM_HIP_TRY(hipLaunchCooperativeKernel(reinterpret_cast<const void*>(&k_local_direct<G>), dim3(uint32_t(grid)),
+                                         dim3(64 * kSuper), args, 0, ctx->stream));
 }
 
 // PWMHIP_NARROW_GATHER=1: one destination word per lane in the stitch.
@@ -3459,6 +3948,10 @@ void build_plain_device(pwmhip_ctx* ctx, const void* d_text, uint64_t n, uint32_
   const bool wide = sup && !wide_gather_disabled();
   const uint32_t wide_nstep = wide_gather_steps();
   const uint64_t wide_wpl = (words_for(n) + 128ull * wide_nstep - 1) / (128ull * wide_nstep);
+  // 2 or 3 byte levels: bucket-0 runs written in place by the local build
+  // (16-chunk units, offsets in one fused launch, the wide stitch)
+  const bool direct = wide && width == 1 && (L == 2 || L == 3) && !h_stream && !na && super_waves() == 16 &&
+                      (units + kTile - 1) / kTile <= 1024 && !direct_disabled();
 
   // group plan (at most 64 / kMaxGroup groups; a fixed array, no heap)
   struct {
@@ -3481,6 +3974,8 @@ void build_plain_device(pwmhip_ctx* ctx, const void* d_text, uint64_t n, uint32_
   const size_t sz_tile = scan_scratch_words(units, nslot_max);
   const size_t sz_tab = std::max<size_t>(1, units * nslot_max);
   uint32_t* wave_first = nullptr;
+  unsigned long long* agg = nullptr;  // direct: published bucket-0 counts per unit
+  uint64_t* bucket0 = nullptr;        // direct: bucket 0's total per sub-level
   auto carve = [&](arena& A, uint64_t*& temp_levels, uint64_t*& temp_payload, uint64_t*& S_ping,
                    uint64_t*& S_pong, uint32_t*& counts, uint64_t*& pref, uint32_t*& ztot, uint64_t*& totals,
                    uint64_t*& gstart, uint64_t*& zeros_dev, unsigned long long*& sym_or, uint64_t*& tiles,
@@ -3505,6 +4000,8 @@ void build_plain_device(pwmhip_ctx* ctx, const void* d_text, uint64_t n, uint32_
     tab_dst = A.take<uint64_t>(sz_tab);
     tab_src = A.take<uint64_t>(sz_tab);
     wave_first = A.take<uint32_t>(std::max<size_t>(1, wide ? size_t(gmax) * wide_wpl : 1));
+    agg = A.take<unsigned long long>(direct ? units : 1);
+    bucket0 = A.take<uint64_t>(kMaxGroup + 1);
   };
   uint64_t *temp_levels, *temp_payload, *S_ping, *S_pong, *pref, *totals, *gstart, *zeros_dev, *tiles;
   uint64_t *nepref, *netot, *nebase, *level_ne, *tab_dst, *tab_src;
@@ -3530,7 +4027,8 @@ void build_plain_device(pwmhip_ctx* ctx, const void* d_text, uint64_t n, uint32_
   carve(na ? na_view : ctx->scratch, temp_levels, temp_payload, S_ping, S_pong, counts, pref, ztot, totals, gstart,
         zeros_dev, sym_or, tiles, nepref, netot, nebase, level_ne, tab_dst, tab_src);
   PWM_HIP_TRY(hipMemsetAsync(sym_or, 0, 8, s));
-  PWM_HIP_TRY(hipMemsetAsync(zeros_dev, 0, 64 * 8, s));
+  PWM_HIP_TRY(hipMemsetAsync(zeros_dev, 0, 64 * 8, s));  // (direct: [63] fault)
+  if (direct) PWM_HIP_TRY(hipMemsetAsync(agg, 0, units * 8, s));
 
   PWM_HIP_TRY(hipEventRecord(ctx->ev[0], s));
   const uint64_t* S_in = nullptr;
@@ -3566,6 +4064,13 @@ void build_plain_device(pwmhip_ctx* ctx, const void* d_text, uint64_t n, uint32_
     k1.lut = na ? na->lut : compress_table(ctx);
     if (h_stream) {
       stream_local_build(ctx, h_stream, n, width, L, k1);
+    } else if (direct) {
+      DirectParams dp{};
+      dp.agg = agg;
+      dp.fault = reinterpret_cast<unsigned long long*>(zeros_dev + 63);
+      for (uint32_t j = 1; j < L; ++j) dp.dst[j] = ws->d_words + ws->level_offset[j];
+      if (L == 2) launch_direct<2>(ctx, k1, dp);
+      else launch_direct<3>(ctx, k1, dp);
     } else if (sup) {
       switch (L) {
         case 2: launch_super<2>(ctx, k1); break;
@@ -3601,6 +4106,7 @@ void build_plain_device(pwmhip_ctx* ctx, const void* d_text, uint64_t n, uint32_
         fr.nslot = nslot;
         fr.tree = tree_direct ? 1 : 0;
         fr.sc_slot0 = nslot;
+        fr.bucket0 = direct ? bucket0 : nullptr;
         hipLaunchKernelGGL(k_runs_fused, dim3(uint32_t(ntiles), nslot), dim3(1024), 0, s, fr);
       } else {
         hipLaunchKernelGGL(k_tile_scan, dim3(nslot), dim3(1024), 0, s, tsum, totals, ntiles);
@@ -3662,7 +4168,12 @@ void build_plain_device(pwmhip_ctx* ctx, const void* d_text, uint64_t n, uint32_
         wp.n = n;
         wp.wpl = wide_wpl;
         wp.nstep = wide_nstep;
+        wp.bucket0 = direct ? bucket0 : nullptr;
         hipLaunchKernelGGL(k_gather_wide, dim3(uint32_t((wide_wpl + 3) / 4), jmax), dim3(256), 0, s, wp);
+        if (direct) {  // entries of a level: at most units x its buckets (2^jmax)
+          const uint64_t emax = nchunks << jmax;
+          hipLaunchKernelGGL(k_bucket0_edges, dim3(uint32_t((emax + 255) / 256), jmax), dim3(256), 0, s, wp);
+        }
       }
     } else if (gi == 0) {
       PWM_HIP_TRY(hipEventRecord(ctx->ev[2], s));
@@ -3685,7 +4196,7 @@ void build_plain_device(pwmhip_ctx* ctx, const void* d_text, uint64_t n, uint32_
     ctx->pinned_words = 128;
   }
   uint64_t* res = na ? na_res : ctx->pinned;
-  PWM_HIP_TRY(hipMemcpyAsync(res, zeros_dev, L * 8, hipMemcpyDeviceToHost, s));
+  PWM_HIP_TRY(hipMemcpyAsync(res, zeros_dev, (direct ? 64 : L) * 8, hipMemcpyDeviceToHost, s));
   PWM_HIP_TRY(hipMemcpyAsync(res + 64, sym_or, 8, hipMemcpyDeviceToHost, s));
   PWM_HIP_TRY(hipStreamSynchronize(s));
   float ms = 0;
@@ -3697,6 +4208,8 @@ void build_plain_device(pwmhip_ctx* ctx, const void* d_text, uint64_t n, uint32_
   ctx->last_ms[2] = ms;
   PWM_HIP_TRY(hipEventElapsedTime(&ms, ctx->ev[0], ctx->ev[3]));
   ctx->last_ms[3] = ms;
+  if (direct && res[63])
+    throw status_error{PWMHIP_ERR_TIMEOUT, "local build: a wait for another unit's counts gave up"};
   if (check_range && L < 64 && (res[64] >> L) != 0)
     throw status_error{PWMHIP_ERR_SYMBOL_RANGE, "symbol >= 2^levels"};
   // trees above one pass: matrix levels with buckets moved into numeric order

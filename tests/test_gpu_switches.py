@@ -25,6 +25,7 @@ SWITCHES = [
     {"PWMHIP_SCATTER_RANKED": "1", "PWMHIP_PASS_UNITS": "1"},
     {"PWMHIP_HUFF_CHUNKS": "1"},
     {"PWMHIP_HUFF_RUN_STITCH": "1"},
+    {"PWMHIP_STITCH_ALL": "1"},
 ]
 
 
