@@ -135,7 +135,7 @@ def rooflines(b_min, phases_ms, total_ms, config, passes=1):
     whole = {"achieved": round(build, 1), "frac": round(build / HBM_PEAK_GBS, 4), "device_ms": round(total_ms, 4)}
     if passes == 1:
         k1 = b_min / (phases_ms[0] / 1e3) / 1e9
-        roof = {"bound": "hbm", "kernel": "local build (k_local_super / k_local_small)", "achieved": round(k1, 1),
+        roof = {"bound": "hbm", "kernel": "local build (k_local_direct / k_local_super / k_local_small)", "achieved": round(k1, 1),
                 "peak": HBM_PEAK_GBS, "unit": "GB/s", "frac": round(k1 / HBM_PEAK_GBS, 4),
                 "traffic": traffic_from_profiles(config), "algorithmic_bytes": int(b_min),
                 "kernel_ms": round(phases_ms[0], 4)}
