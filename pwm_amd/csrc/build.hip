@@ -318,7 +318,58 @@ __device__ __forceinline__ void load_planes_text(const K1Params& p, const void* 
       }
     }
   };
-  if constexpr (sizeof(T) >= 2) {
+  if constexpr (sizeof(T) == 2 || sizeof(T) == 4) {
+    // a full lane at a 16-byte aligned address: its 64 symbols in batches of
+    // 64 bytes, each batch's four 16-byte loads issued together; a plane's
+    // byte of four symbols is picked from the loaded dwords with v_perm
+    // (one for 2-byte symbols, three for 4-byte ones)
+    const T* lp = static_cast<const T*>(text) + e0;
+    if (cnt == 64 && (reinterpret_cast<uintptr_t>(lp) & 15) == 0) {
+      constexpr int SB = 64 / int(sizeof(T));  // symbols per batch
+      const auto g4 = gmem(reinterpret_cast<const uint4*>(lp));
+#pragma unroll 1
+      for (int b = 0; b < 64 / SB; ++b) {
+        uint32_t u[16];
+#pragma unroll
+        for (int i = 0; i < 4; ++i) {
+          const uint4 w = g4[4 * b + i];
+          u[4 * i] = w.x;
+          u[4 * i + 1] = w.y;
+          u[4 * i + 2] = w.z;
+          u[4 * i + 3] = w.w;
+        }
+        uint32_t o = 0;
+#pragma unroll
+        for (int i = 0; i < 16; ++i) o |= u[i];
+        my_or |= sizeof(T) == 2 ? ((o | (o >> 16)) & 0xffffu) : o;
+#pragma unroll
+        for (int g = 0; g < SB / 4; ++g) {
+          uint32_t cur_bi = 0xffffffffu, x4 = 0;
+#pragma unroll
+          for (int k = 0; k < NP; ++k) {
+            if (k < int(p.np)) {
+              const uint32_t sb = p.L - 1 - p.l0 - k;
+              const uint32_t bi = sb >> 3;
+              if (bi != cur_bi) {
+                cur_bi = bi;
+                if constexpr (sizeof(T) == 2) {
+                  // symbols 4g..4g+3 are the halves of u[2g], u[2g + 1]
+                  x4 = __builtin_amdgcn_perm(u[2 * g + 1], u[2 * g], 0x06040200u + bi * 0x01010101u);
+                } else {
+                  const uint32_t lo = __builtin_amdgcn_perm(u[4 * g + 1], u[4 * g], 0x0c0c0400u + bi * 0x0101u);
+                  const uint32_t hi = __builtin_amdgcn_perm(u[4 * g + 3], u[4 * g + 2], 0x0c0c0400u + bi * 0x0101u);
+                  x4 = lo | (hi << 16);
+                }
+              }
+              acc[k] |= uint64_t(gather4(x4, sb & 7)) << (4 * (b * (SB / 4) + g));
+            }
+          }
+        }
+      }
+      return;
+    }
+  }
+  if constexpr (sizeof(T) == 8) {
     // a full lane at a 16-byte aligned address: its 64 symbols in batches of
     // 64 bytes, each batch's four 16-byte loads issued together
     const T* lp = static_cast<const T*>(text) + e0;
@@ -342,9 +393,7 @@ __device__ __forceinline__ void load_planes_text(const K1Params& p, const void* 
 #pragma unroll
           for (int e = 0; e < 4; ++e) {
             const int i = 4 * g + e;  // symbol of the batch
-            if constexpr (sizeof(T) == 2) v[e] = (u[i >> 1] >> (16 * (i & 1))) & 0xffffu;
-            else if constexpr (sizeof(T) == 4) v[e] = u[i];
-            else v[e] = uint64_t(u[2 * i]) | (uint64_t(u[2 * i + 1]) << 32);
+            v[e] = uint64_t(u[2 * i]) | (uint64_t(u[2 * i + 1]) << 32);
           }
           planes4(v, b * (SB / 4) + g);
         }
@@ -1591,9 +1640,9 @@ __global__ __launch_bounds__(64 * kSuper) void k_local_direct(K1Params p, Direct
 // chunks (the single-text super-chunk build per unit).  Per-unit metadata
 // (first symbol, symbols left in the text, level-0 word, text) comes from a
 // table, so that a unit needs one round of independent loads, issued a unit
-// ahead.  A workgroup takes units from the queue one at a time, the next
-// one a unit ahead so that its text is prefetched; zero counts are reduced
-// per unit from the waves' bucket totals (one atomic per level and unit).
+// ahead.  Workgroup b takes units b, b + G, b + 2G, ..., the next one's
+// text prefetched during a unit; zero counts are reduced per unit from the
+// waves' bucket totals (one atomic per level and unit).
 struct UnitMeta {
   uint64_t text;  // address of the unit's first symbol
   uint64_t rem;   // symbols of the text from the unit's first on
@@ -1634,12 +1683,7 @@ __global__ __launch_bounds__(64 * WAVES) void k_local_super_batch(K1Params p, co
   __shared__ uint64_t img[NSUB][kUnitWords];
   __shared__ uint32_t wt[WAVES][NF];
   __shared__ unsigned long long red_or;
-  __shared__ uint64_t s_unit[3];
-  if (threadIdx.x == 0) {
-    red_or = 0;
-    s_unit[0] = atomicAdd(p.work, 1u);
-    s_unit[1] = atomicAdd(p.work, 1u);
-  }
+  if (threadIdx.x == 0) red_or = 0;
   for (uint32_t i = threadIdx.x; i < NSUB * kUnitWords; i += 64 * WAVES) (&img[0][0])[i] = 0;
   const int lane = threadIdx.x & 63;
   const int wid = threadIdx.x >> 6;
@@ -1667,12 +1711,15 @@ __global__ __launch_bounds__(64 * WAVES) void k_local_super_batch(K1Params p, co
     for (int i = 0; i < 4; ++i) d[i] = src[i];
   };
   uint4 pf[4];
-  uint64_t u = s_unit[0], un = s_unit[1];
+  // units blockIdx.x + k gridDim.x: every workgroup runs about the same
+  // number of units (their sizes differ little on average), and no queue
+  // atomic is waited for at every step
+  const uint64_t G_ = gridDim.x;
+  uint64_t u = blockIdx.x, un = u + G_;
   UnitMeta cur = get(u), nxt = get(un);
   if (u < U && full(cur)) load_chunk(cur, pf);
 
   while (u < U) {
-    if (threadIdx.x == 0) s_unit[2] = atomicAdd(p.work, 1u);  // read after the first barrier
     const uint32_t Ec = chunk_syms(cur);
     const uint32_t nw = (Ec + 63) >> 6;
     const int cnt = max(0, min(64, int(Ec) - 64 * lane));
@@ -1739,7 +1786,7 @@ __global__ __launch_bounds__(64 * WAVES) void k_local_super_batch(K1Params p, co
     lds_sync();  // every wave's totals are there; the image was cleared
 
     // the unit after the next: its metadata loads overlap the rest of this unit
-    const uint64_t unn = s_unit[2];
+    const uint64_t unn = un + G_;
     const UnitMeta nn = get(unn);
 
     uint32_t stot[NF], spre[NQ];
