@@ -2720,13 +2720,19 @@ __device__ __forceinline__ void gather_range_wide(const uint64_t* tab_dst, const
       el = __popcll(__ballot(rc.bd < gend)) - 1;
     }
     // steps inside one cached run (whose end is cached too); a step wholly
-    // inside a bucket-0 run written in place is done
-    uint32_t single = 0, inplace = 0;
-    int64_t dl[kSteps];
+    // inside a bucket-0 run written in place is done; a step that one run
+    // boundary crosses (neither run in place) is a shift copy per lane from
+    // the run its words lie in, and the lane whose words hold the boundary
+    // also loads the second run's bits
+    uint32_t single = 0, inplace = 0, two = 0;
+    int64_t dl[kSteps], dlb[kSteps];
+    uint64_t bnd[kSteps];  // two-run steps: the second run's first bit
 #pragma unroll
     for (int k = 0; k < int(kSteps); ++k) {
       const uint64_t sw0 = (gw0 + 128ull * k) << 6, sw1 = min<uint64_t>(sw0 + 8192, n);
       dl[k] = 0;
+      dlb[k] = 0;
+      bnd[k] = 0;
       if (sw0 < n && el < 63) {
         const uint32_t a = __popcll(__ballot(rc.bd <= sw0)) - 1;
         const uint32_t b = __popcll(__ballot(rc.bd < sw1)) - 1;
@@ -2739,29 +2745,68 @@ __device__ __forceinline__ void gather_range_wide(const uint64_t* tab_dst, const
             single |= 1u << k;
             dl[k] = int64_t(readlane64(rc.bs, a)) - int64_t(d0);
           }
+        } else if (b == a + 1) {
+          const uint64_t d0 = readlane64(rc.bd, a);
+          if (d0 >= T0) {
+            const uint64_t B = readlane64(rc.bd, b);
+            two |= 1u << k;
+            bnd[k] = B;
+            dl[k] = int64_t(readlane64(rc.bs, a)) - int64_t(d0);
+            dlb[k] = int64_t(readlane64(rc.bs, b)) - int64_t(B);
+          }
         }
       }
     }
+    const uint32_t shift_copy = single | two;
     u64x2_u lo2[kSteps];
-    uint64_t hi1[kSteps];
+    uint64_t hi1[kSteps], xb[kSteps][3];
 #pragma unroll
     for (int k = 0; k < int(kSteps); ++k) {
       const uint64_t ws = ((gw0 + 128ull * k) << 6) + 128ull * lane;
-      if (((single >> k) & 1) && ws < n) {
-        const uint64_t q = uint64_t(dl[k] + int64_t(ws)) >> 6;
+      if (((shift_copy >> k) & 1) && ws < n) {
+        const bool second = ((two >> k) & 1) && ws >= bnd[k];  // the lane's words lie in the second run
+        const uint64_t q = uint64_t((second ? dlb[k] : dl[k]) + int64_t(ws)) >> 6;
         lo2[k] = ld_stream2(src + q);
         hi1[k] = gmem(src)[q + 2];
+        if (((two >> k) & 1) && ws < bnd[k] && bnd[k] < ws + 128) {
+          // the boundary lane: the second run's bits for destination bits
+          // ws.. start at source bit pb >= (its first source bit) - 127
+          const int64_t pb = dlb[k] + int64_t(ws);
+          const int64_t qb = pb >> 6;  // >= -2
+          xb[k][0] = qb >= 0 ? gmem(src)[qb] : 0ull;
+          xb[k][1] = qb + 1 >= 0 ? gmem(src)[qb + 1] : 0ull;
+          xb[k][2] = gmem(src)[qb + 2];
+        }
       }
     }
 #pragma unroll
     for (int k = 0; k < int(kSteps); ++k) {
       const uint64_t ws = ((gw0 + 128ull * k) << 6) + 128ull * lane;
-      if (((single >> k) & 1) && ws < n) {
-        const uint32_t sh = uint32_t(uint64_t(dl[k] + int64_t(ws)) & 63);
+      if (((shift_copy >> k) & 1) && ws < n) {
+        const bool second = ((two >> k) & 1) && ws >= bnd[k];
+        const uint32_t sh = uint32_t(uint64_t((second ? dlb[k] : dl[k]) + int64_t(ws)) & 63);
         uint64_t v0 = lo2[k].x, v1 = lo2[k].y;
         if (sh) {
           v0 = (v0 << sh) | (lo2[k].y >> (64 - sh));
           v1 = (v1 << sh) | (hi1[k] >> (64 - sh));
+        }
+        if (((two >> k) & 1) && ws < bnd[k] && bnd[k] < ws + 128) {
+          // destination bits [ws, B) from the first run, [B, ws + 128) from the second
+          const uint32_t shb = uint32_t(uint64_t(dlb[k] + int64_t(ws)) & 63);
+          uint64_t b0 = xb[k][0], b1 = xb[k][1];
+          if (shb) {
+            b0 = (b0 << shb) | (xb[k][1] >> (64 - shb));
+            b1 = (b1 << shb) | (xb[k][2] >> (64 - shb));
+          }
+          const uint32_t cut = uint32_t(bnd[k] - ws);  // 1..127
+          if (cut < 64) {
+            const uint64_t m = range_mask(0, cut);
+            v0 = (v0 & m) | (b0 & ~m);
+            v1 = b1;
+          } else {
+            const uint64_t m = range_mask(0, cut - 64);
+            v1 = (v1 & m) | (b1 & ~m);
+          }
         }
         uint64_t* d = dst + (ws >> 6);
         if (ws + 128 <= n) {
@@ -2781,7 +2826,7 @@ __device__ __forceinline__ void gather_range_wide(const uint64_t* tab_dst, const
     }
 #pragma unroll 1
     for (uint32_t k = 0; k < kSteps; ++k) {
-      if (((single | inplace) >> k) & 1) continue;
+      if (((shift_copy | inplace) >> k) & 1) continue;
       const uint64_t w0 = gw0 + 128ull * k;
       if ((w0 << 6) >= n) break;
       gather_window(src, dst, rc, w0, stage, T0);
