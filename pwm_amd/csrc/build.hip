@@ -2162,48 +2162,89 @@ __global__ __launch_bounds__(64 * scatter_seg_waves<TO>()) void k_scatter_seg(Sc
   const auto out = gmem(static_cast<TO*>(p.out));
   const uint32_t E = uint32_t(min<uint64_t>(kChunkSyms, p.n - c * kChunkSyms));
   const int m = max(0, min(64, int(E) - 64 * lane));  // this lane's symbols
-  const uint64_t low = p.sh >= 64 ? ~0ull : ((1ull << p.sh) - 1);  // bits kept
   // the lane's symbols: 16-byte loads when the whole segment is there and aligned
-  union {
-    uint4 v[V];
-    TI x[64];
-  } seg;
+  // (kept as dwords, PD symbols each, the first in the low bits)
+  constexpr int PD = 4 / int(sizeof(TI));  // symbols per dword
+  constexpr uint32_t TB = 8 * sizeof(TI);  // bits per symbol
+  constexpr int ND = 64 / PD;              // dwords per lane
+  uint32_t d[ND];
   if (m == 64 && (reinterpret_cast<uintptr_t>(src) & 15) == 0) {
     const auto g = gmem(reinterpret_cast<const uint4*>(src));
 #pragma unroll
-    for (int i = 0; i < V; ++i) seg.v[i] = g[i];
+    for (int i = 0; i < V; ++i) {
+      const uint4 w = g[i];
+      d[4 * i] = w.x;
+      d[4 * i + 1] = w.y;
+      d[4 * i + 2] = w.z;
+      d[4 * i + 3] = w.w;
+    }
   } else {
     const auto g = gmem(src);
 #pragma unroll
-    for (int i = 0; i < 64; ++i) seg.x[i] = i < m ? g[i] : TI(0);
+    for (int j = 0; j < ND; ++j) {
+      uint32_t x = 0;
+#pragma unroll
+      for (int e = 0; e < PD; ++e)
+        if (PD * j + e < m) x |= uint32_t(g[PD * j + e]) << (e * TB);
+      d[j] = x;
+    }
   }
-  auto bucket = [&](TI x) -> uint32_t {
-    const uint32_t f = uint32_t(uint64_t(x) >> p.sh) & uint32_t(NQ - 1);
-    return __builtin_bitreverse32(f) >> (32 - G);
+  // Counters are kept per field value f (the G bits at sh); bucket q (the
+  // chunk's order) is the bit reversal of f, applied only when the counters
+  // are scanned.  A symbol's field and its kept low bits are one bit-field
+  // extract each from the dword that holds it (sh + G <= the symbol's bits;
+  // an extract of width 0 is 0).
+  const uint32_t sh = p.sh;  // < 32
+  auto field = [&](int i) -> uint32_t { return __builtin_amdgcn_ubfe(d[i / PD], (i % PD) * TB + sh, G); };
+  auto kept = [&](int i) -> TO { return TO(__builtin_amdgcn_ubfe(d[i / PD], (i % PD) * TB, sh)); };
+  auto lds_add = [](uint32_t* a) {
+    return __hip_atomic_fetch_add(a, 1u, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_WORKGROUP);
   };
-  uint32_t* cl = &ctr[wid][0][lane];  // this lane's counter of bucket q at cl[64 q]
+  uint32_t* cl = &ctr[wid][0][lane];  // this lane's counter of field value f at cl[64 f]
 #pragma unroll
   for (int q = 0; q < NQ; ++q) cl[64 * q] = 0;
+  // whole chunks (wave-uniform) without a test per symbol
+  const bool whole = E == kChunkSyms;
+  if (whole) {
 #pragma unroll
-  for (int i = 0; i < 64; ++i)
-    if (i < m) atomicAdd(cl + 64 * bucket(seg.x[i]), 1u);
+    for (int i = 0; i < 64; ++i) lds_add(cl + 64 * field(i));
+  } else {
+#pragma unroll
+    for (int i = 0; i < 64; ++i)
+      if (i < m) lds_add(cl + 64 * field(i));
+  }
   wave_sync();
   // counters -> positions: bucket start in the chunk + count in lanes below
   uint32_t run = 0, lst = 0, tot = 0;
 #pragma unroll
   for (int q = 0; q < NQ; ++q) {
-    const uint32_t v = cl[64 * q];
+    const int f = int(__builtin_bitreverse32(uint32_t(q)) >> (32 - G));  // (a constant)
+    const uint32_t v = cl[64 * f];
     const uint32_t inc = wave_incl_scan(v);
-    cl[64 * q] = run + inc - v;
+    cl[64 * f] = run + inc - v;
     const uint32_t t = __builtin_amdgcn_readlane(inc, 63);
     lst = lane == q ? run : lst;
     tot = lane == q ? t : tot;
     run += t;
   }
   wave_sync();
+  if (whole) {
+    // eight positions taken before their symbols are stored, so that the
+    // returning adds overlap (a lane's adds to one counter still return
+    // in order: one wave's LDS operations execute in program order)
 #pragma unroll
-  for (int i = 0; i < 64; ++i)
-    if (i < m) st[atomicAdd(cl + 64 * bucket(seg.x[i]), 1u)] = TO(uint64_t(seg.x[i]) & low);
+    for (int i0 = 0; i0 < 64; i0 += 8) {
+      uint32_t pos[8];
+#pragma unroll
+      for (int k = 0; k < 8; ++k) pos[k] = lds_add(cl + 64 * field(i0 + k));
+#pragma unroll
+      for (int k = 0; k < 8; ++k) st[pos[k]] = kept(i0 + k);
+    }
+  } else {
+#pragma unroll
+    for (int i = 0; i < 64; ++i)
+      if (i < m) st[lds_add(cl + 64 * field(i))] = kept(i);
+  }
   wave_sync();  // the staged chunk is complete
   const uint64_t gpos = lane < NQ ? p.dstpos[c * NQ + lane] : 0ull;
   // (the bucket sizes the positions were made from bound every run)
