@@ -2256,7 +2256,15 @@ __global__ __launch_bounds__(64 * scatter_seg_waves<TO>()) void k_scatter_seg(Sc
     const uint32_t s0 = __builtin_amdgcn_readlane(lst, q);
     const uint64_t dst = (uint64_t(__builtin_amdgcn_readlane(uint32_t(gpos >> 32), q)) << 32) |
                          __builtin_amdgcn_readlane(uint32_t(gpos), q);
-    for (uint32_t i = lane; i < len; i += 64) out[dst + i] = st[s0 + i];
+    // a uniform base and a 32-bit lane offset: scalar-base stores, LDS reads
+    // at immediate offsets; whole rounds of 64 without a test per lane
+    const auto o = out + dst;
+    const TO* s = st + s0;
+    const uint32_t nfull = len >> 6;
+    uint32_t i = uint32_t(lane);
+#pragma unroll 4
+    for (uint32_t k = 0; k < nfull; ++k, i += 64) o[i] = s[i];
+    if (i < len) o[i] = s[i];
   }
 }
 
