@@ -119,6 +119,66 @@ __global__ void k_gather_runs(uint64_t* lv, uint64_t stride, const uint64_t* wor
   dst[w] = v;
 }
 
+// The wide stitch's run tables for trees of up to 2^8 leaves, from the leaf
+// histogram, in one workgroup (no host round trip).  Level l's runs are its
+// non-empty l-bit prefixes p in numeric (tree) order; run r of level l is
+// entry (1 << l) - 2 + r, with its tree start (the symbols whose prefix is
+// below p) and its matrix start (the symbols whose prefix, read with its l
+// bits reversed, is below p's reversal).  tab: [0, kMaxGroup) runs per
+// level, then K tree starts, then K matrix starts.
+constexpr uint32_t kTableLeaves = 256;
+
+__global__ __launch_bounds__(kTableLeaves) void k_wt_tables(const unsigned long long* hist, uint32_t L,
+                                                            uint64_t* tab) {
+  __shared__ uint64_t h[kTableLeaves], pcs[kTableLeaves], ms[kTableLeaves], sa[kTableLeaves];
+  const uint32_t t = threadIdx.x;
+  const uint32_t K = 1u << L;
+  h[t] = t < K ? hist[t] : 0ull;
+  for (uint32_t i = t; i < kMaxGroup + 2 * K; i += kTableLeaves) tab[i] = 0;
+  __syncthreads();
+  // exclusive scan over the workgroup's threads (every thread calls it)
+  auto excl_scan = [&](uint64_t v) -> uint64_t {
+    sa[t] = v;
+    __syncthreads();
+    for (uint32_t d = 1; d < kTableLeaves; d <<= 1) {
+      const uint64_t x = t >= d ? sa[t - d] : 0ull;
+      __syncthreads();
+      sa[t] += x;
+      __syncthreads();
+    }
+    const uint64_t r = sa[t] - v;
+    __syncthreads();  // sa is free again
+    return r;
+  };
+  uint64_t* ne = tab;
+  uint64_t* td = tab + kMaxGroup;
+  uint64_t* ts = td + K;
+  for (uint32_t l = 1; l < L; ++l) {
+    const uint32_t nb = 1u << l, w = L - l;  // prefixes; leaves per prefix: 2^w
+    uint64_t c = 0;
+    if (t < nb)
+      for (uint32_t i = 0; i < (1u << w); ++i) c += h[(t << w) + i];
+    pcs[t] = c;
+    __syncthreads();
+    // matrix order: position q holds the prefix whose l bits reversed are q
+    const uint32_t rq = t < nb ? __builtin_bitreverse32(t) >> (32 - l) : 0u;
+    const uint64_t mstart = excl_scan(t < nb ? pcs[rq] : 0ull);
+    if (t < nb) ms[rq] = mstart;
+    const uint64_t tstart = excl_scan(c);
+    const uint64_t r = excl_scan(c ? 1ull : 0ull);
+    __syncthreads();  // ms is complete
+    if (t < nb && c) {
+      td[nb - 2 + r] = tstart;
+      ts[nb - 2 + r] = ms[t];
+    }
+    if (t == nb - 1) ne[l] = r + (c ? 1 : 0);
+    __syncthreads();  // pcs and ms are read
+  }
+}
+
+void host_tables_gather(pwmhip_ctx* ctx, const unsigned long long* d_hist, uint64_t n, uint32_t L, pwmhip_ws* ws,
+                        uint64_t* d_tab, uint64_t* d_lv, uint64_t stride, uint64_t tab);
+
 }  // namespace
 
 void wm_to_wt_device(pwmhip_ctx* ctx, const void* d_text, uint64_t n, uint32_t width, uint32_t L,
@@ -155,6 +215,50 @@ void wm_to_wt_device(pwmhip_ctx* ctx, const void* d_text, uint64_t n, uint32_t w
     default: hipLaunchKernelGGL(k_leaf_hist<uint64_t>, dim3(grid), dim3(256), 0, st, (const uint64_t*)d_text, n, L, d_hist); break;
   }
   PWM_HIP_TRY(hipGetLastError());
+  const uint64_t nw = words_for(n);
+  if (wide) {
+    // (wide: L <= kMaxGroup, so K <= kTableLeaves) the run tables on the device
+    hipLaunchKernelGGL(k_wt_tables, dim3(1), dim3(kTableLeaves), 0, st, d_hist, L, d_tab);
+    PWM_HIP_TRY(hipGetLastError());
+    WideStitch w{};
+    w.level_ne = d_tab;
+    w.tab_dst = d_tab + kMaxGroup;
+    w.tab_src = d_tab + kMaxGroup + K;
+    w.first = d_first;
+    w.src = ws->d_words + ws->level_offset[1];
+    w.src_stride = stride;
+    for (uint32_t j = 1; j < L; ++j) {
+      w.dst[j] = d_lv + uint64_t(j - 1) * stride;
+      w.nbits[j] = n;
+    }
+    w.units = 1;
+    w.nsub = L - 1;
+    launch_wide_stitch(ctx, w);
+  } else {
+    host_tables_gather(ctx, d_hist, n, L, ws, d_tab, d_lv, stride, tab);
+  }
+  // the tree levels replace the matrix levels in place (one copy when
+  // levels 1.. are contiguous at one stride)
+  if (contiguous) {
+    PWM_HIP_TRY(hipMemcpyAsync(ws->d_words + ws->level_offset[1], d_lv, uint64_t(L - 1) * stride * 8,
+                               hipMemcpyDeviceToDevice, st));
+  } else {
+    for (uint32_t l = 1; l < L; ++l)
+      PWM_HIP_TRY(hipMemcpyAsync(ws->d_words + ws->level_offset[l], d_lv + uint64_t(l - 1) * stride, nw * 8,
+                                 hipMemcpyDeviceToDevice, st));
+  }
+  PWM_HIP_TRY(hipStreamSynchronize(st));  // (the scratch is free again on return)
+}
+
+namespace {
+
+// Trees whose levels are not one contiguous block, or of more than
+// kMaxGroup levels: the run tables on the host from the leaf histogram,
+// then one thread per destination word (k_gather_runs).
+void host_tables_gather(pwmhip_ctx* ctx, const unsigned long long* d_hist, uint64_t n, uint32_t L, pwmhip_ws* ws,
+                        uint64_t* d_tab, uint64_t* d_lv, uint64_t stride, uint64_t tab) {
+  hipStream_t st = ctx->stream;
+  const uint64_t K = 1ull << L;
   std::vector<uint64_t> pc(K);
   PWM_HIP_TRY(hipMemcpyAsync(pc.data(), d_hist, K * 8, hipMemcpyDeviceToHost, st));
   PWM_HIP_TRY(hipStreamSynchronize(st));
@@ -166,11 +270,6 @@ void wm_to_wt_device(pwmhip_ctx* ctx, const void* d_text, uint64_t n, uint32_t w
   std::vector<uint64_t> off(L), nr(L);
   all.reserve(tab);
   all.assign(3 * uint64_t(L), 0);
-  // the wide stitch's layout: entries per level, destination starts, source starts
-  std::vector<uint64_t> wtab(wide ? kMaxGroup + 2 * K : 0, 0);
-  uint64_t* w_ne = wtab.data();
-  uint64_t* w_dst = w_ne + kMaxGroup;
-  uint64_t* w_src = w_dst + K;
   for (uint32_t l = L - 1; l >= 1; --l) {
     std::vector<uint64_t> up(1ull << l);
     for (uint64_t q = 0; q < up.size(); ++q) up[q] = pc[2 * q] + pc[2 * q + 1];
@@ -197,57 +296,23 @@ void wm_to_wt_device(pwmhip_ctx* ctx, const void* d_text, uint64_t n, uint32_t w
     ss.push_back(0);
     off[l] = all.size();
     nr[l] = ds.size() - 1;
-    if (wide) {
-      w_ne[l] = nr[l];
-      for (uint64_t r = 0; r < nr[l]; ++r) {
-        w_dst[(1ull << l) - 2 + r] = ds[r];
-        w_src[(1ull << l) - 2 + r] = ss[r];
-      }
-      continue;
-    }
     all.insert(all.end(), ds.begin(), ds.end());
     all.insert(all.end(), ss.begin(), ss.end());
   }
+  for (uint32_t l = 1; l < L; ++l) {
+    all[3 * l] = off[l];
+    all[3 * l + 1] = nr[l];
+    all[3 * l + 2] = ws->level_offset[l];
+  }
+  PWM_HIP_TRY(hipMemcpyAsync(d_tab, all.data(), all.size() * 8, hipMemcpyHostToDevice, st));
   const uint64_t nw = words_for(n);
-  if (wide) {
-    PWM_HIP_TRY(hipMemcpyAsync(d_tab, wtab.data(), wtab.size() * 8, hipMemcpyHostToDevice, st));
-    WideStitch w{};
-    w.level_ne = d_tab;
-    w.tab_dst = d_tab + kMaxGroup;
-    w.tab_src = d_tab + kMaxGroup + K;
-    w.first = d_first;
-    w.src = ws->d_words + ws->level_offset[1];
-    w.src_stride = stride;
-    for (uint32_t j = 1; j < L; ++j) {
-      w.dst[j] = d_lv + uint64_t(j - 1) * stride;
-      w.nbits[j] = n;
-    }
-    w.units = 1;
-    w.nsub = L - 1;
-    launch_wide_stitch(ctx, w);
-  } else {
-    for (uint32_t l = 1; l < L; ++l) {
-      all[3 * l] = off[l];
-      all[3 * l + 1] = nr[l];
-      all[3 * l + 2] = ws->level_offset[l];
-    }
-    PWM_HIP_TRY(hipMemcpyAsync(d_tab, all.data(), all.size() * 8, hipMemcpyHostToDevice, st));
-    hipLaunchKernelGGL(k_gather_runs, dim3(uint32_t((nw + 255) / 256), L - 1), dim3(256), 0, st, d_lv, stride,
-                       ws->d_words, d_tab, n);
-    PWM_HIP_TRY(hipGetLastError());
-  }
-  // the tree levels replace the matrix levels in place (one copy when
-  // levels 1.. are contiguous at one stride)
-  if (contiguous) {
-    PWM_HIP_TRY(hipMemcpyAsync(ws->d_words + ws->level_offset[1], d_lv, uint64_t(L - 1) * stride * 8,
-                               hipMemcpyDeviceToDevice, st));
-  } else {
-    for (uint32_t l = 1; l < L; ++l)
-      PWM_HIP_TRY(hipMemcpyAsync(ws->d_words + ws->level_offset[l], d_lv + uint64_t(l - 1) * stride, nw * 8,
-                                 hipMemcpyDeviceToDevice, st));
-  }
+  hipLaunchKernelGGL(k_gather_runs, dim3(uint32_t((nw + 255) / 256), L - 1), dim3(256), 0, st, d_lv, stride,
+                     ws->d_words, d_tab, n);
+  PWM_HIP_TRY(hipGetLastError());
   PWM_HIP_TRY(hipStreamSynchronize(st));  // (the host run table)
 }
+
+}  // namespace
 
 void copy_bits_device(hipStream_t st, uint64_t* dst, const uint64_t* src, uint64_t dst_off,
                       uint64_t src_off, uint64_t len) {

@@ -1286,7 +1286,6 @@ struct DirectParams {
   uint64_t* dst[kMaxGroup];   // final sub-level j at dst[j]
 };
 
-constexpr uint32_t kAggField = 32;
 constexpr unsigned long long kAggFlag = 0x8000000080000000ull;  // in both dwords: a dword read alone is whole
 constexpr uint32_t kGapEntries = 256;      // gap entries loaded a step ahead (grid <= 257)
 constexpr uint32_t kSpinLimit = 1u << 16;  // reloads of one entry before its wait gives up
