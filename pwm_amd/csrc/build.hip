@@ -477,21 +477,27 @@ __device__ __forceinline__ void planes_from_bytes(const K1Params& p, const uint4
   my_or |= (o | (o >> 8) | (o >> 16) | (o >> 24)) & 0xffu;
 #pragma unroll
   for (int k = 0; k < NP; ++k) {
+    const uint32_t sft = p.L - 1 - p.l0 - k;
+    const uint32_t m = 0x01010101u << sft;
+    // a plane whose bit none of the lane's 64 bytes has is zero (small
+    // alphabets: a rare symbol's bit is clear in most lanes)
     if (k < int(p.np)) {
-      // 8 symbols -> one byte of the plane: masked bit s of each byte times
-      // its place value, two 4-byte dot products chained through the
-      // accumulator; the result is scaled by 2^s.
-      const uint32_t sft = p.L - 1 - p.l0 - k;
-      const uint32_t m = 0x01010101u << sft;
-      uint32_t b[8];
+      if (!(o & m)) {
+        acc[k] = 0;
+      } else {
+        // 8 symbols -> one byte of the plane: masked bit s of each byte
+        // times its place value, two 4-byte dot products chained through
+        // the accumulator; the result is scaled by 2^s.
+        uint32_t b[8];
 #pragma unroll
-      for (int q = 0; q < 8; ++q)
-        b[q] = __builtin_amdgcn_udot4(x[2 * q + 1] & m, 0x80402010u,
-                                      __builtin_amdgcn_udot4(x[2 * q] & m, 0x08040201u, 0u, false), false) >>
-               sft;
-      const uint32_t lo = b[0] | (b[1] << 8) | (b[2] << 16) | (b[3] << 24);
-      const uint32_t hi = b[4] | (b[5] << 8) | (b[6] << 16) | (b[7] << 24);
-      acc[k] = (uint64_t(hi) << 32) | lo;
+        for (int q = 0; q < 8; ++q)
+          b[q] = __builtin_amdgcn_udot4(x[2 * q + 1] & m, 0x80402010u,
+                                        __builtin_amdgcn_udot4(x[2 * q] & m, 0x08040201u, 0u, false), false) >>
+                 sft;
+        const uint32_t lo = b[0] | (b[1] << 8) | (b[2] << 16) | (b[3] << 24);
+        const uint32_t hi = b[4] | (b[5] << 8) | (b[6] << 16) | (b[7] << 24);
+        acc[k] = (uint64_t(hi) << 32) | lo;
+      }
     }
   }
 }
