@@ -68,14 +68,21 @@ __global__ __launch_bounds__(256) void k_leaf_hist(const T* text, uint64_t n, ui
   const uint64_t step = uint64_t(gridDim.x) * blockDim.x;
   const uint64_t nv = (reinterpret_cast<uintptr_t>(text) & 15) == 0 ? n / PER : 0;
   const uint4* tv = reinterpret_cast<const uint4*>(text);
-  for (uint64_t i = gid; i < nv; i += step) {
+  // four 16-byte loads issued together per round (one round trip, not four)
+  constexpr int U = 4;
+  for (uint64_t i = gid; i < nv; i += U * step) {
     union {
       uint4 v;
       T x[PER];
-    } u;
-    u.v = tv[i];
+    } u[U];
 #pragma unroll
-    for (uint32_t k = 0; k < PER; ++k) add(u.x[k]);
+    for (int r = 0; r < U; ++r)
+      if (i + r * step < nv) u[r].v = tv[i + r * step];
+#pragma unroll
+    for (int r = 0; r < U; ++r)
+      if (i + r * step < nv)
+#pragma unroll
+        for (uint32_t k = 0; k < PER; ++k) add(u[r].x[k]);
   }
   for (uint64_t i = nv * PER + gid; i < n; i += step) add(text[i]);
   __syncthreads();
@@ -120,7 +127,7 @@ __global__ void k_gather_runs(uint64_t* lv, uint64_t stride, const uint64_t* wor
 }
 
 // The wide stitch's run tables for trees of up to 2^8 leaves, from the leaf
-// histogram, in one workgroup (no host round trip).  Level l's runs are its
+// histogram, in one wave (no host round trip).  Level l's runs are its
 // non-empty l-bit prefixes p in numeric (tree) order; run r of level l is
 // entry (1 << l) - 2 + r, with its tree start (the symbols whose prefix is
 // below p) and its matrix start (the symbols whose prefix, read with its l
@@ -128,51 +135,60 @@ __global__ void k_gather_runs(uint64_t* lv, uint64_t stride, const uint64_t* wor
 // level, then K tree starts, then K matrix starts.
 constexpr uint32_t kTableLeaves = 256;
 
-__global__ __launch_bounds__(kTableLeaves) void k_wt_tables(const unsigned long long* hist, uint32_t L,
-                                                            uint64_t* tab) {
-  __shared__ uint64_t h[kTableLeaves], pcs[kTableLeaves], ms[kTableLeaves], sa[kTableLeaves];
-  const uint32_t t = threadIdx.x;
+__device__ __forceinline__ uint64_t wave_excl_scan64(uint64_t v) {
+  const uint32_t lane = threadIdx.x & 63;
+  uint64_t x = v;
+#pragma unroll
+  for (int d = 1; d < 64; d <<= 1) {
+    const uint64_t y = __shfl_up(x, d, 64);
+    if (lane >= uint32_t(d)) x += y;
+  }
+  return x - v;
+}
+
+__global__ __launch_bounds__(64) void k_wt_tables(const unsigned long long* hist, uint32_t L, uint64_t* tab) {
+  // prefix counts as a heap: level l's prefix p at pc[(1 << l) + p], leaves at level L
+  __shared__ uint64_t pc[2 * kTableLeaves], ms[kTableLeaves];
+  const uint32_t lane = threadIdx.x;
   const uint32_t K = 1u << L;
-  h[t] = t < K ? hist[t] : 0ull;
-  for (uint32_t i = t; i < kMaxGroup + 2 * K; i += kTableLeaves) tab[i] = 0;
+  for (uint32_t i = lane; i < K; i += 64) pc[K + i] = hist[i];
+  for (uint32_t i = lane; i < kMaxGroup + 2 * K; i += 64) tab[i] = 0;
   __syncthreads();
-  // exclusive scan over the workgroup's threads (every thread calls it)
-  auto excl_scan = [&](uint64_t v) -> uint64_t {
-    sa[t] = v;
+  for (uint32_t l = L - 1; l >= 1; --l) {
+    const uint32_t nb = 1u << l;
+    for (uint32_t q = lane; q < nb; q += 64) pc[nb + q] = pc[2 * nb + 2 * q] + pc[2 * nb + 2 * q + 1];
     __syncthreads();
-    for (uint32_t d = 1; d < kTableLeaves; d <<= 1) {
-      const uint64_t x = t >= d ? sa[t - d] : 0ull;
-      __syncthreads();
-      sa[t] += x;
-      __syncthreads();
-    }
-    const uint64_t r = sa[t] - v;
-    __syncthreads();  // sa is free again
-    return r;
-  };
+  }
   uint64_t* ne = tab;
   uint64_t* td = tab + kMaxGroup;
   uint64_t* ts = td + K;
+  // each lane holds elements 2 lane and 2 lane + 1 of a level (at most 128)
+  const uint32_t e0 = 2 * lane, e1 = 2 * lane + 1;
   for (uint32_t l = 1; l < L; ++l) {
-    const uint32_t nb = 1u << l, w = L - l;  // prefixes; leaves per prefix: 2^w
-    uint64_t c = 0;
-    if (t < nb)
-      for (uint32_t i = 0; i < (1u << w); ++i) c += h[(t << w) + i];
-    pcs[t] = c;
-    __syncthreads();
-    // matrix order: position q holds the prefix whose l bits reversed are q
-    const uint32_t rq = t < nb ? __builtin_bitreverse32(t) >> (32 - l) : 0u;
-    const uint64_t mstart = excl_scan(t < nb ? pcs[rq] : 0ull);
-    if (t < nb) ms[rq] = mstart;
-    const uint64_t tstart = excl_scan(c);
-    const uint64_t r = excl_scan(c ? 1ull : 0ull);
+    const uint32_t nb = 1u << l;
+    const uint64_t* c = pc + nb;
+    auto rev = [&](uint32_t q) { return __builtin_bitreverse32(q) >> (32 - l); };
+    // matrix order: element q holds the prefix whose l bits reversed are q
+    const uint64_t a0 = e0 < nb ? c[rev(e0)] : 0ull, a1 = e1 < nb ? c[rev(e1)] : 0ull;
+    const uint64_t mx = wave_excl_scan64(a0 + a1);
+    if (e0 < nb) ms[rev(e0)] = mx;
+    if (e1 < nb) ms[rev(e1)] = mx + a0;
+    // tree order, and the run index among the non-empty prefixes
+    const uint64_t b0 = e0 < nb ? c[e0] : 0ull, b1 = e1 < nb ? c[e1] : 0ull;
+    const uint64_t tx = wave_excl_scan64(b0 + b1);
+    const uint32_t n0 = b0 ? 1u : 0u, n1 = b1 ? 1u : 0u;
+    const uint64_t rx = wave_excl_scan64(n0 + n1);
     __syncthreads();  // ms is complete
-    if (t < nb && c) {
-      td[nb - 2 + r] = tstart;
-      ts[nb - 2 + r] = ms[t];
+    if (b0) {
+      td[nb - 2 + rx] = tx;
+      ts[nb - 2 + rx] = ms[e0];
     }
-    if (t == nb - 1) ne[l] = r + (c ? 1 : 0);
-    __syncthreads();  // pcs and ms are read
+    if (b1) {
+      td[nb - 2 + rx + n0] = tx + b0;
+      ts[nb - 2 + rx + n0] = ms[e1];
+    }
+    if (lane == 63) ne[l] = rx + n0 + n1;
+    __syncthreads();  // ms is read
   }
 }
 
@@ -218,7 +234,7 @@ void wm_to_wt_device(pwmhip_ctx* ctx, const void* d_text, uint64_t n, uint32_t w
   const uint64_t nw = words_for(n);
   if (wide) {
     // (wide: L <= kMaxGroup, so K <= kTableLeaves) the run tables on the device
-    hipLaunchKernelGGL(k_wt_tables, dim3(1), dim3(kTableLeaves), 0, st, d_hist, L, d_tab);
+    hipLaunchKernelGGL(k_wt_tables, dim3(1), dim3(64), 0, st, d_hist, L, d_tab);
     PWM_HIP_TRY(hipGetLastError());
     WideStitch w{};
     w.level_ne = d_tab;
