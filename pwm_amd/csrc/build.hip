@@ -4316,9 +4316,8 @@ void build_plain_device(pwmhip_ctx* ctx, const void* d_text, uint64_t n, uint32_
         wp.nstep = wide_nstep;
         wp.bucket0 = direct ? bucket0 : nullptr;
         hipLaunchKernelGGL(k_gather_wide, dim3(uint32_t((wide_wpl + 3) / 4), jmax), dim3(256), 0, s, wp);
-        if (direct) {  // entries of a level: at most units x its buckets (2^jmax)
-          const uint64_t emax = nchunks << jmax;
-          hipLaunchKernelGGL(k_bucket0_edges, dim3(uint32_t((emax + 255) / 256), jmax), dim3(256), 0, s, wp);
+        if (direct) {  // bucket-0 runs: the first entries of a level, at most one per unit
+          hipLaunchKernelGGL(k_bucket0_edges, dim3(uint32_t((nchunks + 255) / 256), jmax), dim3(256), 0, s, wp);
         }
       }
     } else if (gi == 0) {
