@@ -296,6 +296,10 @@ __device__ __forceinline__ uint32_t gather4(uint32_t x4, uint32_t s) {
   return __builtin_amdgcn_udot4((x4 >> s) & 0x01010101u, 0x08040201u, 0u, false);
 }
 
+template <int NP>
+__device__ __forceinline__ void planes_from_bytes(const K1Params& p, const uint4* d, uint64_t* acc,
+                                                  uint64_t& my_or);
+
 template <int NP, typename T>
 __device__ __forceinline__ void load_planes_text(const K1Params& p, const void* text, uint64_t e0, int cnt,
                                                  uint64_t* acc, uint64_t& my_or) {
@@ -402,11 +406,39 @@ __device__ __forceinline__ void load_planes_text(const K1Params& p, const void* 
     }
   }
   if constexpr (sizeof(T) == 1) {
-    // bytes (partial chunks, unaligned texts): four groups of 16 bytes, each
-    // group's loads issued together (one 16-byte load when aligned), so a
-    // lane waits for 4 round trips rather than 16
     const uint8_t* lp = static_cast<const uint8_t*>(text) + e0;
     const bool a16 = (reinterpret_cast<uintptr_t>(lp) & 15) == 0;
+    if (p.L - 1 - p.l0 < 8) {
+      // every plane's bit lies in the byte: the lane's 64 bytes (zero past
+      // the chunk's end) with all loads issued together, then the planes as
+      // for whole chunks
+      uint4 d[4];
+#pragma unroll
+      for (int g = 0; g < 4; ++g) {
+        if (cnt >= 16 * g + 16 && a16) {
+          d[g] = *gmem(reinterpret_cast<const uint4*>(lp + 16 * g));
+        } else {
+          uint32_t w[4];
+#pragma unroll
+          for (int i = 0; i < 4; ++i) {
+            const int q = 4 * g + i;
+            if (cnt >= 4 * q + 4) {
+              w[i] = *gmem(reinterpret_cast<const uint32_t*>(lp + 4 * q));
+            } else {
+              uint32_t x = 0;
+              for (int e = 0; e < 4; ++e)
+                if (4 * q + e < cnt) x |= uint32_t(tx[e0 + 4 * q + e]) << (8 * e);
+              w[i] = x;
+            }
+          }
+          d[g] = make_uint4(w[0], w[1], w[2], w[3]);
+        }
+      }
+      planes_from_bytes<NP>(p, d, acc, my_or);
+      return;
+    }
+    // (symbol bits above the byte: planes of zeros among them) four groups
+    // of 16 bytes, each group's loads issued together
 #pragma unroll 1
     for (int g = 0; g < 4; ++g) {
       uint32_t w[4];
