@@ -3089,6 +3089,7 @@ struct WideParams {
   uint64_t nbits[kMaxGroup]; // bits of sub-level j when its own (0: n)
   const uint64_t* bucket0;   // or nullptr: [j] = bucket 0's total of sub-level j, whose
                              // runs' whole words the local build wrote in place
+  uint32_t edge_blocks;      // with bucket0: the grid's first blocks build the shared words
 };
 
 // First table entry of every wave of the wide stitch: the last entry whose
@@ -3114,32 +3115,20 @@ __global__ __launch_bounds__(256) void k_wave_starts(WideParams p) {
 
 // With bucket-0 runs written in place (p.bucket0), the waves whose words all
 // lie below bucket 0's end have nothing left but the words two runs share:
-// they exit, and k_bucket0_edges builds those words.
+// they exit, and the grid's first blocks build those words (bucket0_edge).
 __device__ __forceinline__ uint64_t bucket0_skip_words(const WideParams& p, uint32_t j) {
   const uint64_t wpw = 128ull * p.nstep;  // words per wave
   return p.bucket0 ? (p.bucket0[j] >> 6) / wpw * wpw : 0;
-}
-
-__global__ __launch_bounds__(256) void k_gather_wide(WideParams p) {
-  __shared__ unsigned long long stage[4][64];
-  const uint32_t j = blockIdx.y + 1;
-  const uint64_t wave = uint64_t(blockIdx.x) * 4 + (threadIdx.x >> 6);
-  if (wave >= p.wpl) return;  // whole waves; no barrier below
-  if ((wave + 1) * 128ull * p.nstep <= bucket0_skip_words(p, j)) return;
-  const uint64_t toff = uint64_t((1u << j) - 2) * p.units;
-  gather_range_wide(p.tab_dst + toff, p.tab_src + toff, p.level_ne[j], p.src + (j - 1) * p.src_stride, p.dst[j],
-                    p.nbits[j] ? p.nbits[j] : p.n, wave * 128 * p.nstep, p.nstep,
-                    p.first[(j - 1) * p.wpl + wave], stage[threadIdx.x >> 6], p.bucket0 ? p.bucket0[j] : 0);
 }
 
 // The words below the skipped waves' end that bucket-0 runs share (those
 // not wholly inside one run, which the local build wrote in place): one
 // thread per run, which builds the word holding the run's end when the run
 // also holds that word's first bit, from the following runs as far as the
-// word goes.  Every other word there lies inside one run.
-__global__ __launch_bounds__(256) void k_bucket0_edges(WideParams p) {
-  const uint32_t j = blockIdx.y + 1;
-  const uint64_t e = uint64_t(blockIdx.x) * 256 + threadIdx.x;
+// word goes.  Every other word there lies inside one run.  Run e of
+// sub-level j (bucket-0 runs are a level's first entries, at most one per
+// unit).
+__device__ __forceinline__ void bucket0_edge(const WideParams& p, uint32_t j, uint64_t e) {
   const uint64_t NE = p.level_ne[j];
   if (e >= NE) return;
   const uint64_t toff = uint64_t((1u << j) - 2) * p.units;
@@ -3164,6 +3153,25 @@ __global__ __launch_bounds__(256) void k_bucket0_edges(WideParams p) {
     s0 = ts[ee];
   }
   p.dst[j][w] = v;
+}
+
+// With bucket-0 runs in place, blocks [0, edge_blocks) of each sub-level
+// build the words two bucket-0 runs share (one thread per run); they run
+// while the blocks of the skipped waves are dispatched and exit.
+__global__ __launch_bounds__(256) void k_gather_wide(WideParams p) {
+  __shared__ unsigned long long stage[4][64];
+  const uint32_t j = blockIdx.y + 1;
+  if (blockIdx.x < p.edge_blocks) {
+    bucket0_edge(p, j, uint64_t(blockIdx.x) * 256 + threadIdx.x);
+    return;
+  }
+  const uint64_t wave = uint64_t(blockIdx.x - p.edge_blocks) * 4 + (threadIdx.x >> 6);
+  if (wave >= p.wpl) return;  // whole waves; no barrier below
+  if ((wave + 1) * 128ull * p.nstep <= bucket0_skip_words(p, j)) return;
+  const uint64_t toff = uint64_t((1u << j) - 2) * p.units;
+  gather_range_wide(p.tab_dst + toff, p.tab_src + toff, p.level_ne[j], p.src + (j - 1) * p.src_stride, p.dst[j],
+                    p.nbits[j] ? p.nbits[j] : p.n, wave * 128 * p.nstep, p.nstep,
+                    p.first[(j - 1) * p.wpl + wave], stage[threadIdx.x >> 6], p.bucket0 ? p.bucket0[j] : 0);
 }
 
 __global__ __launch_bounds__(256) void k_gather(GatherParams p) {
@@ -4501,10 +4509,10 @@ void build_plain_device(pwmhip_ctx* ctx, const void* d_text, uint64_t n, uint32_
         wp.wpl = wide_wpl;
         wp.nstep = wide_nstep;
         wp.bucket0 = direct ? bucket0 : nullptr;
-        hipLaunchKernelGGL(k_gather_wide, dim3(uint32_t((wide_wpl + 3) / 4), jmax), dim3(256), 0, s, wp);
-        if (direct) {  // bucket-0 runs: the first entries of a level, at most one per unit
-          hipLaunchKernelGGL(k_bucket0_edges, dim3(uint32_t((nchunks + 255) / 256), jmax), dim3(256), 0, s, wp);
-        }
+        // (bucket-0 runs: the first entries of a level, at most one per unit)
+        wp.edge_blocks = direct ? uint32_t((nchunks + 255) / 256) : 0u;
+        hipLaunchKernelGGL(k_gather_wide, dim3(wp.edge_blocks + uint32_t((wide_wpl + 3) / 4), jmax), dim3(256), 0, s,
+                           wp);
       }
     } else if (gi == 0) {
       PWM_HIP_TRY(hipEventRecord(ctx->ev[2], s));
