@@ -331,6 +331,11 @@ __device__ __forceinline__ void load_planes_text(const K1Params& p, const void* 
     if (cnt == 64 && (reinterpret_cast<uintptr_t>(lp) & 15) == 0) {
       constexpr int SB = 64 / int(sizeof(T));  // symbols per batch
       const auto g4 = gmem(reinterpret_cast<const uint4*>(lp));
+      // every plane's bit in one byte of the symbol (a pass's 4 levels
+      // usually are): that byte of four symbols with one v_perm, then the
+      // byte planes' chained dot products (8 symbols per plane byte)
+      const uint32_t s0 = p.L - 1 - p.l0;  // plane 0's bit
+      const bool one_byte = (s0 >> 3) == ((s0 + 1 - p.np) >> 3);
 #pragma unroll 1
       for (int b = 0; b < 64 / SB; ++b) {
         uint32_t u[16];
@@ -346,6 +351,37 @@ __device__ __forceinline__ void load_planes_text(const K1Params& p, const void* 
 #pragma unroll
         for (int i = 0; i < 16; ++i) o |= u[i];
         my_or |= sizeof(T) == 2 ? ((o | (o >> 16)) & 0xffffu) : o;
+        if (one_byte) {
+          const uint32_t bi = s0 >> 3;
+          uint32_t xq[SB / 4];  // byte bi of symbols 4g..4g+3 of the batch
+#pragma unroll
+          for (int g = 0; g < SB / 4; ++g) {
+            if constexpr (sizeof(T) == 2) {
+              xq[g] = __builtin_amdgcn_perm(u[2 * g + 1], u[2 * g], 0x06040200u + bi * 0x01010101u);
+            } else {
+              const uint32_t lo = __builtin_amdgcn_perm(u[4 * g + 1], u[4 * g], 0x0c0c0400u + bi * 0x0101u);
+              const uint32_t hi = __builtin_amdgcn_perm(u[4 * g + 3], u[4 * g + 2], 0x0c0c0400u + bi * 0x0101u);
+              xq[g] = lo | (hi << 16);
+            }
+          }
+#pragma unroll
+          for (int k = 0; k < NP; ++k) {
+            if (k < int(p.np)) {
+              const uint32_t sft = (s0 - k) & 7;
+              const uint32_t m = 0x01010101u << sft;
+              uint32_t part = 0;  // the plane's SB bits of this batch
+#pragma unroll
+              for (int h = 0; h < SB / 8; ++h)
+                part |= (__builtin_amdgcn_udot4(xq[2 * h + 1] & m, 0x80402010u,
+                                                __builtin_amdgcn_udot4(xq[2 * h] & m, 0x08040201u, 0u, false),
+                                                false) >>
+                         sft)
+                        << (8 * h);
+              acc[k] |= uint64_t(part) << (SB * b);
+            }
+          }
+          continue;
+        }
 #pragma unroll
         for (int g = 0; g < SB / 4; ++g) {
           uint32_t cur_bi = 0xffffffffu, x4 = 0;
