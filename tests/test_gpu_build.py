@@ -169,3 +169,30 @@ def test_passes_over_units_vs_oracle(w, L):
         ws.free()
     finally:
         del os.environ["PWMHIP_PASS_UNITS"]
+
+
+@pytest.mark.gpu
+@pytest.mark.parametrize("w,L,ranked", [(8, 17, False), (2, 9, True), (8, 17, True)])
+def test_passes_over_units_reorder_forms_vs_oracle(w, L, ranked):
+    """Passes over units with the reorder that ranks 64 symbols per step
+    (8-byte symbols always, others with PWMHIP_SCATTER_RANKED=1): its chunk
+    positions come from the unit positions as in the counting sort's."""
+    import os
+    rng = np.random.default_rng(7100 + 10 * w + L + ranked)
+    os.environ["PWMHIP_PASS_UNITS"] = "1"
+    if ranked:
+        os.environ["PWMHIP_SCATTER_RANKED"] = "1"
+    try:
+        for n in [4097, 65537, 3 * 65536 + 4099, 300001]:
+            for kind in ("uniform", "skew"):
+                t = _random_case(rng, w, L, n, kind)
+                for tree in (False, True):
+                    o = O.naive(t, L, tree)
+                    ws = pwm_amd.build(t, L, "pps", tree, check_range=True)
+                    _assert_same(ws, o.levels, ("units", w, L, ranked, n, kind, tree))
+                    if not tree:
+                        assert np.array_equal(ws.zeros(), o.zeros)
+                    ws.free()
+    finally:
+        del os.environ["PWMHIP_PASS_UNITS"]
+        os.environ.pop("PWMHIP_SCATTER_RANKED", None)
