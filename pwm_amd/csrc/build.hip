@@ -2820,12 +2820,15 @@ __device__ __forceinline__ void gather_range(const uint64_t* tab_dst, const uint
 // run boundary; four steps issue their loads together.  Other steps go
 // through gather_window (64-word windows, one word per lane).
 // T0 > 0: the words wholly inside bucket-0 runs (runs starting below T0) were
-// written in place by the local build and are skipped.
+// written in place by the local build and are skipped.  STEPS: steps whose
+// loads go out together (the batched stitch takes 2: fewer registers, five
+// waves a SIMD instead of three).
+template <uint32_t STEPS>
 __device__ __forceinline__ void gather_range_wide(const uint64_t* tab_dst, const uint64_t* tab_src, uint64_t NE,
                                                   const uint64_t* src, uint64_t* dst, uint64_t n, uint64_t W0,
                                                   uint32_t nstep, uint64_t first, unsigned long long* stage,
                                                   uint64_t T0 = 0) {
-  constexpr uint32_t kSteps = 4;
+  constexpr uint32_t kSteps = STEPS;
   const uint32_t lane = threadIdx.x & 63;
   const uint64_t X = W0 << 6;
   if (X >= n || NE == 0) return;
@@ -2966,151 +2969,6 @@ __device__ __forceinline__ void gather_range_wide(const uint64_t* tab_dst, const
   }
 }
 
-// The batches' form of gather_range_wide: of the steps that run boundaries
-// cross, only those that one boundary crosses are shift copies, from two
-// wave-uniform offsets (fewer registers: the batched stitch keeps four waves
-// a SIMD; with the per-lane search it ran 0.57 ms against 0.49 ms on 20,000
-// DNA texts).  Two-word lanes, 128-word steps, four steps' loads together;
-// other steps go through gather_window (64-word windows, one word per lane).
-// T0 > 0: the words wholly inside bucket-0 runs (runs starting below T0) were
-// written in place by the local build and are skipped.
-__device__ __forceinline__ void gather_range_wide2(const uint64_t* tab_dst, const uint64_t* tab_src, uint64_t NE,
-                                                  const uint64_t* src, uint64_t* dst, uint64_t n, uint64_t W0,
-                                                  uint32_t nstep, uint64_t first, unsigned long long* stage,
-                                                  uint64_t T0 = 0) {
-  constexpr uint32_t kSteps = 4;
-  const uint32_t lane = threadIdx.x & 63;
-  const uint64_t X = W0 << 6;
-  if (X >= n || NE == 0) return;
-  RunCache rc{tab_dst, tab_src, NE, n, first, 0, 0};  // first: last entry with dst <= X
-  rc.load(lane);
-  for (uint32_t g0 = 0; g0 < nstep; g0 += kSteps) {
-    const uint64_t gw0 = W0 + 128ull * g0;
-    if ((gw0 << 6) >= n) break;
-    const uint64_t gend = min<uint64_t>((gw0 + 128ull * kSteps) << 6, n);
-    uint32_t ef = __popcll(__ballot(rc.bd <= (gw0 << 6))) - 1;
-    uint32_t el = __popcll(__ballot(rc.bd < gend)) - 1;
-    if (el == 63 && ef > 0) {
-      rc.base += ef;
-      rc.load(lane);
-      el = __popcll(__ballot(rc.bd < gend)) - 1;
-    }
-    // steps inside one cached run (whose end is cached too); a step wholly
-    // inside a bucket-0 run written in place is done; a step that one run
-    // boundary crosses (neither run in place) is a shift copy per lane from
-    // the run its words lie in, and the lane whose words hold the boundary
-    // also loads the second run's bits
-    uint32_t single = 0, inplace = 0, two = 0;
-    int64_t dl[kSteps], dlb[kSteps];
-    uint64_t bnd[kSteps];  // two-run steps: the second run's first bit
-#pragma unroll
-    for (int k = 0; k < int(kSteps); ++k) {
-      const uint64_t sw0 = (gw0 + 128ull * k) << 6, sw1 = min<uint64_t>(sw0 + 8192, n);
-      dl[k] = 0;
-      dlb[k] = 0;
-      bnd[k] = 0;
-      if (sw0 < n && el < 63) {
-        const uint32_t a = __popcll(__ballot(rc.bd <= sw0)) - 1;
-        const uint32_t b = __popcll(__ballot(rc.bd < sw1)) - 1;
-        if (a == b) {
-          const uint64_t d0 = readlane64(rc.bd, a);
-          if (d0 < T0) {
-            if (readlane64(rc.bd, a + 1) >= sw0 + 8192) inplace |= 1u << k;
-            // (a bucket-0 run ending inside the step: the level's last, partial word)
-          } else {
-            single |= 1u << k;
-            dl[k] = int64_t(readlane64(rc.bs, a)) - int64_t(d0);
-          }
-        } else if (b == a + 1) {
-          const uint64_t d0 = readlane64(rc.bd, a);
-          if (d0 >= T0) {
-            const uint64_t B = readlane64(rc.bd, b);
-            two |= 1u << k;
-            bnd[k] = B;
-            dl[k] = int64_t(readlane64(rc.bs, a)) - int64_t(d0);
-            dlb[k] = int64_t(readlane64(rc.bs, b)) - int64_t(B);
-          }
-        }
-      }
-    }
-    const uint32_t shift_copy = single | two;
-    u64x2_u lo2[kSteps];
-    uint64_t hi1[kSteps], xb[kSteps][3];
-#pragma unroll
-    for (int k = 0; k < int(kSteps); ++k) {
-      const uint64_t ws = ((gw0 + 128ull * k) << 6) + 128ull * lane;
-      if (((shift_copy >> k) & 1) && ws < n) {
-        const bool second = ((two >> k) & 1) && ws >= bnd[k];  // the lane's words lie in the second run
-        const uint64_t q = uint64_t((second ? dlb[k] : dl[k]) + int64_t(ws)) >> 6;
-        lo2[k] = ld_stream2(src + q);
-        hi1[k] = gmem(src)[q + 2];
-        if (((two >> k) & 1) && ws < bnd[k] && bnd[k] < ws + 128) {
-          // the boundary lane: the second run's bits for destination bits
-          // ws.. start at source bit pb >= (its first source bit) - 127
-          const int64_t pb = dlb[k] + int64_t(ws);
-          const int64_t qb = pb >> 6;  // >= -2
-          xb[k][0] = qb >= 0 ? gmem(src)[qb] : 0ull;
-          xb[k][1] = qb + 1 >= 0 ? gmem(src)[qb + 1] : 0ull;
-          xb[k][2] = gmem(src)[qb + 2];
-        }
-      }
-    }
-#pragma unroll
-    for (int k = 0; k < int(kSteps); ++k) {
-      const uint64_t ws = ((gw0 + 128ull * k) << 6) + 128ull * lane;
-      if (((shift_copy >> k) & 1) && ws < n) {
-        const bool second = ((two >> k) & 1) && ws >= bnd[k];
-        const uint32_t sh = uint32_t(uint64_t((second ? dlb[k] : dl[k]) + int64_t(ws)) & 63);
-        uint64_t v0 = lo2[k].x, v1 = lo2[k].y;
-        if (sh) {
-          v0 = (v0 << sh) | (lo2[k].y >> (64 - sh));
-          v1 = (v1 << sh) | (hi1[k] >> (64 - sh));
-        }
-        if (((two >> k) & 1) && ws < bnd[k] && bnd[k] < ws + 128) {
-          // destination bits [ws, B) from the first run, [B, ws + 128) from the second
-          const uint32_t shb = uint32_t(uint64_t(dlb[k] + int64_t(ws)) & 63);
-          uint64_t b0 = xb[k][0], b1 = xb[k][1];
-          if (shb) {
-            b0 = (b0 << shb) | (xb[k][1] >> (64 - shb));
-            b1 = (b1 << shb) | (xb[k][2] >> (64 - shb));
-          }
-          const uint32_t cut = uint32_t(bnd[k] - ws);  // 1..127
-          if (cut < 64) {
-            const uint64_t m = range_mask(0, cut);
-            v0 = (v0 & m) | (b0 & ~m);
-            v1 = b1;
-          } else {
-            const uint64_t m = range_mask(0, cut - 64);
-            v1 = (v1 & m) | (b1 & ~m);
-          }
-        }
-        uint64_t* d = dst + (ws >> 6);
-        if (ws + 128 <= n) {
-          u64x2_u o;
-          o.x = v0;
-          o.y = v1;
-          st_stream2(d, o);
-        } else {  // the level's end: bits from n on are zero
-          if (ws + 64 < n) {
-            gmem(d)[0] = v0;
-            gmem(d)[1] = v1 & range_mask(0, uint32_t(n - ws - 64));
-          } else {
-            gmem(d)[0] = v0 & range_mask(0, uint32_t(n - ws));
-          }
-        }
-      }
-    }
-#pragma unroll 1
-    for (uint32_t k = 0; k < kSteps; ++k) {
-      if (((shift_copy | inplace) >> k) & 1) continue;
-      const uint64_t w0 = gw0 + 128ull * k;
-      if ((w0 << 6) >= n) break;
-      gather_window(src, dst, rc, w0, stage, T0);
-      if (((w0 + 64) << 6) < n) gather_window(src, dst, rc, w0 + 64, stage, T0);
-    }
-  }
-}
-
 // Every sub-level of a pass in one grid (blockIdx.y = sub-level - 1).
 struct WideParams {
   const uint64_t* tab_dst;   // level j's entries from ((1 << j) - 2) * units
@@ -3205,7 +3063,7 @@ __global__ __launch_bounds__(256) void k_gather_wide(WideParams p) {
   if (wave >= p.wpl) return;  // whole waves; no barrier below
   if ((wave + 1) * 128ull * p.nstep <= bucket0_skip_words(p, j)) return;
   const uint64_t toff = uint64_t((1u << j) - 2) * p.units;
-  gather_range_wide(p.tab_dst + toff, p.tab_src + toff, p.level_ne[j], p.src + (j - 1) * p.src_stride, p.dst[j],
+  gather_range_wide<4>(p.tab_dst + toff, p.tab_src + toff, p.level_ne[j], p.src + (j - 1) * p.src_stride, p.dst[j],
                     p.nbits[j] ? p.nbits[j] : p.n, wave * 128 * p.nstep, p.nstep,
                     p.first[(j - 1) * p.wpl + wave], stage[threadIdx.x >> 6], p.bucket0 ? p.bucket0[j] : 0);
 }
@@ -3292,7 +3150,7 @@ __global__ __launch_bounds__(256) void k_gather_batch_wide(BatchGatherParams p) 
     lo += uint64_t(__popcll(__ballot(pred))) * step;
     hi = min(hi, lo + step);
   }
-  gather_range_wide2(td, p.tab_src + toff, NE, p.src + (p.j - 1) * p.src_stride, p.out + p.out0[t] + p.j * p.stride[t],
+  gather_range_wide<2>(td, p.tab_src + toff, NE, p.src + (p.j - 1) * p.src_stride, p.out + p.out0[t] + p.j * p.stride[t],
                     n, W0, p.nwin, lo, stage[threadIdx.x >> 6]);
 }
 
