@@ -1242,7 +1242,8 @@ __global__ __launch_bounds__(64 * WAVES) void k_local_super(K1Params p) {
       uint32_t lo = 0, hi = 0;
 #pragma unroll
       for (int q = 0; q < NQ; ++q) {
-        const uint32_t v = __popcll(M[q] & ~P0[G - 1]) | (uint32_t(__popcll(M[q] & P0[G - 1])) << 16);
+        const uint32_t c1 = uint32_t(__popcll(M[q] & P0[G - 1]));
+        const uint32_t v = (f[q] - c1) | (c1 << 16);  // (f[q]: all of bucket q)
         const uint32_t t = __builtin_amdgcn_readlane(wave_incl_scan(v), 63);
         lo = lane == q ? (t & 0xffffu) : lo;
         hi = lane == q ? (t >> 16) : hi;
