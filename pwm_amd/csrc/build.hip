@@ -235,6 +235,8 @@ struct K1Params {
   uint32_t* chunk_sc;        // super-chunk passes followed by a reorder: every
                              // chunk's counts of the 2^G buckets of the order
                              // after the pass, [c * 2^G + q]
+  uint8_t* lane_sc;          // or nullptr: every lane's counts of the G-bit field
+                             // values of its 64 symbols, byte (c * 64 + lane) * 16 + f
   const uint8_t* lut;        // 64 KiB compress table in device memory
 };
 
@@ -1240,6 +1242,7 @@ __global__ __launch_bounds__(64 * WAVES) void k_local_super(K1Params p) {
       // bucket q of the last sub-level splits by the last level's bit into
       // next-order buckets q and q + NQ (counted in pairs, 16 bits each)
       uint32_t lo = 0, hi = 0;
+      uint32_t lw[4] = {0, 0, 0, 0};  // this lane's counts, byte f: field value f
 #pragma unroll
       for (int q = 0; q < NQ; ++q) {
         const uint32_t c1 = uint32_t(__popcll(M[q] & P0[G - 1]));
@@ -1247,7 +1250,14 @@ __global__ __launch_bounds__(64 * WAVES) void k_local_super(K1Params p) {
         const uint32_t t = __builtin_amdgcn_readlane(wave_incl_scan(v), 63);
         lo = lane == q ? (t & 0xffffu) : lo;
         hi = lane == q ? (t >> 16) : hi;
+        // next-order bucket r holds the symbols whose G-bit field is r reversed
+        const int f0 = int(__builtin_bitreverse32(uint32_t(q)) >> (32 - G));
+        const int f1 = int(__builtin_bitreverse32(uint32_t(q + NQ)) >> (32 - G));
+        lw[f0 >> 2] |= (f[q] - c1) << (8 * (f0 & 3));
+        lw[f1 >> 2] |= c1 << (8 * (f1 & 3));
       }
+      if (p.lane_sc && c < p.nchunks)
+        *gmem(reinterpret_cast<uint4*>(p.lane_sc + (c * 64 + lane) * 16)) = make_uint4(lw[0], lw[1], lw[2], lw[3]);
       if (lane < NQ) {
         wsc[wid][lane] = lo;
         wsc[wid][lane + NQ] = hi;
@@ -2141,6 +2151,8 @@ struct ScatterParams {
   const uint32_t* counts;    // [c * nslot + slot0 + q]: the chunk's bucket sizes
   uint32_t nslot, slot0;
   uint64_t nchunks;
+  const uint8_t* lane_sc;    // or nullptr: each lane's field-value counts, as the
+                             // pass's local build wrote them (K1Params::lane_sc)
 };
 
 template <typename TO>
@@ -2236,6 +2248,9 @@ __global__ __launch_bounds__(64 * scatter_seg_waves<TO>()) void k_scatter_seg(Sc
   const auto out = gmem(static_cast<TO*>(p.out));
   const uint32_t E = uint32_t(min<uint64_t>(kChunkSyms, p.n - c * kChunkSyms));
   const int m = max(0, min(64, int(E) - 64 * lane));  // this lane's symbols
+  // this lane's field-value counts when the local build left them
+  uint4 lsc = make_uint4(0u, 0u, 0u, 0u);
+  if (p.lane_sc) lsc = *gmem(reinterpret_cast<const uint4*>(p.lane_sc + (c * 64 + lane) * 16));
   // the lane's symbols: 16-byte loads when the whole segment is there and aligned
   // (kept as dwords, PD symbols each, the first in the low bits)
   constexpr int PD = 4 / int(sizeof(TI));  // symbols per dword
@@ -2275,17 +2290,23 @@ __global__ __launch_bounds__(64 * scatter_seg_waves<TO>()) void k_scatter_seg(Sc
     return __hip_atomic_fetch_add(a, 1u, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_WORKGROUP);
   };
   uint32_t* cl = &ctr[wid][0][lane];  // this lane's counter of field value f at cl[64 f]
-#pragma unroll
-  for (int q = 0; q < NQ; ++q) cl[64 * q] = 0;
   // whole chunks (wave-uniform) without a test per symbol
   const bool whole = E == kChunkSyms;
-  if (whole) {
+  if (p.lane_sc) {  // counted by the pass's local build
+    const uint32_t lw[4] = {lsc.x, lsc.y, lsc.z, lsc.w};
 #pragma unroll
-    for (int i = 0; i < 64; ++i) lds_add(cl + 64 * field(i));
+    for (int q = 0; q < NQ; ++q) cl[64 * q] = (lw[q >> 2] >> (8 * (q & 3))) & 0xffu;
   } else {
 #pragma unroll
-    for (int i = 0; i < 64; ++i)
-      if (i < m) lds_add(cl + 64 * field(i));
+    for (int q = 0; q < NQ; ++q) cl[64 * q] = 0;
+    if (whole) {
+#pragma unroll
+      for (int i = 0; i < 64; ++i) lds_add(cl + 64 * field(i));
+    } else {
+#pragma unroll
+      for (int i = 0; i < 64; ++i)
+        if (i < m) lds_add(cl + 64 * field(i));
+    }
   }
   wave_sync();
   // counters -> positions: bucket start in the chunk + count in lanes below
@@ -3922,6 +3943,7 @@ void build_passes_device(pwmhip_ctx* ctx, const void* d_text, uint64_t n, uint32
     v.push_back(A.take<uint64_t>(sup ? std::max<size_t>(1, units_max << gmax) : 1));  // 16 unit positions
     v.push_back(A.take<uint32_t>(sup ? std::max<size_t>(1, nchunks << gmax) : 1));  // 17 chunk counts
     v.push_back(A.take<uint32_t>(sup ? std::max<size_t>(1, size_t(gmax) * wide_wpl) : 1));  // 18 wave firsts
+    v.push_back(A.take<uint8_t>(sup && P > 1 ? std::max<size_t>(1, nchunks * 1024) : 1));  // 19 lane counts
   };
   std::vector<void*> b;
   {
@@ -3971,6 +3993,7 @@ void build_passes_device(pwmhip_ctx* ctx, const void* d_text, uint64_t n, uint32
     const uint64_t usyms = sup ? pw * kChunkSyms : kChunkSyms;
     if (sup) {
       k1.chunk_sc = static_cast<uint32_t*>(b[17]);
+      k1.lane_sc = last ? nullptr : static_cast<uint8_t*>(b[19]);
       if (pw == 16) launch_pass_units<16>(ctx, k1, g, !last);
       else launch_pass_units<8>(ctx, k1, g, !last);
     } else {
@@ -4064,6 +4087,7 @@ void build_passes_device(pwmhip_ctx* ctx, const void* d_text, uint64_t n, uint32
         sp.nslot = sup ? (1u << g) : nslot;
         sp.slot0 = sup ? 0u : (1u << g) - 2u;
         sp.nchunks = nchunks;
+        sp.lane_sc = sup ? static_cast<const uint8_t*>(b[19]) : nullptr;
         const uint32_t wout = width_for_bits(R - g);
         launch_scatter(s, cw, wout, g, sp);
         cur = sp.out;
