@@ -103,6 +103,34 @@ def test_batch_dna_lognormal_texts_vs_oracle():
 
 
 @pytest.mark.gpu
+@pytest.mark.parametrize("waves", [None, "8", "16"])
+def test_batch_unit_sizes_vs_oracle(waves, monkeypatch):
+    """Batches over units of 8 and of 16 chunks: few long texts (16 by
+    default), many short ones (8 by default), and either size forced with
+    PWMHIP_SUPER_WAVES; every structure against the oracle, matrix and tree."""
+    if waves is not None:
+        monkeypatch.setenv("PWMHIP_SUPER_WAVES", waves)
+    rng = np.random.default_rng(0xB16 + (0 if waves is None else int(waves)))
+    shapes = [
+        np.array([70001, 900000, 4096 * 16, 123457, 600000, 4096 * 8 + 5, 700001, 65536 * 3], np.int64),
+        np.clip(rng.lognormal(np.log(40000), 1.0, 120), 32768, 400000).astype(np.int64),
+    ]
+    for si, sizes in enumerate(shapes):
+        big = pwm_amd.generate("dna", 0xB17 + si, int(sizes.sum()))
+        offs = np.concatenate([[0], np.cumsum(sizes)])
+        texts = [big[offs[i]:offs[i + 1]] for i in range(len(sizes))]
+        for tree in (False, True):
+            got = pwm_amd.build_batch(texts, 3, "pps", tree)
+            for i, t in enumerate(texts):
+                o = O.naive(t, 3, tree)
+                for l, (x, y) in enumerate(zip(got[i].bvs(), o.levels)):
+                    assert np.array_equal(x, y), (waves, si, tree, i, l)
+                if not tree:
+                    assert np.array_equal(got[i].zeros(), o.zeros), (waves, si, i)
+            del got
+
+
+@pytest.mark.gpu
 def test_reuse_of_batch_views_is_refused():
     """build(reuse=<a structure of a batch>) must not take over (or free) the
     batch's shared block: the new structure gets its own memory and every
