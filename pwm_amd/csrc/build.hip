@@ -1435,8 +1435,10 @@ __global__ __launch_bounds__(64 * kSuper) void k_local_direct(K1Params p, Direct
       const uint64_t w = wf + threadIdx.x;  // (a run holds at most 1024 whole words)
       if (w < wl) {
         const uint32_t m = uint32_t(w - w0);  // 1 <= m when s > 0, m <= 1023
-        uint64_t v = img[b][j][m];
-        if (s) v = (v << s) | (img[b][j][m - 1] >> (64 - s));
+        // (no branch on s: with s = 0 the second word is the first again
+        // and the shifts leave it as it is)
+        const uint64_t v0 = img[b][j][m], v1 = img[b][j][m - (s ? 1u : 0u)];
+        const uint64_t v = (v0 << s) | (v1 >> ((64 - s) & 63));
         st_stream(d.dst[j + 1] + w, brev64(v));
       }
     }
