@@ -1425,7 +1425,9 @@ __global__ __launch_bounds__(64 * kSuper) void k_local_direct(K1Params p, Direct
     }
   };
   // words wholly inside the bucket-0 runs [D[j], D[j] + cl[j]) of sub-levels
-  // j + 1, from LDS image b, one per thread
+  // j + 1, from LDS image b, one per thread (to dstp[j]: d.dst[j + 1] held in
+  // vector registers, set below)
+  uint64_t dstp[NSUB];
   auto write_in_place = [&](int b, const uint64_t* D, const uint32_t* cl) {
 #pragma unroll
     for (int j = 0; j < NSUB; ++j) {
@@ -1439,7 +1441,7 @@ __global__ __launch_bounds__(64 * kSuper) void k_local_direct(K1Params p, Direct
         // and the shifts leave it as it is)
         const uint64_t v0 = img[b][j][m], v1 = img[b][j][m - (s ? 1u : 0u)];
         const uint64_t v = (v0 << s) | (v1 >> ((64 - s) & 63));
-        st_stream(d.dst[j + 1] + w, brev64(v));
+        st_stream(reinterpret_cast<uint64_t*>(dstp[j]) + w, brev64(v));
       }
     }
   };
@@ -1456,6 +1458,11 @@ __global__ __launch_bounds__(64 * kSuper) void k_local_direct(K1Params p, Direct
   // (values touched once a step live in vector registers: the scalar file
   // is full, and scalar spills cost vector instructions every step)
   auto in_vgpr = [](auto& x) { asm volatile("" : "+v"(x)); };
+#pragma unroll
+  for (int j = 0; j < NSUB; ++j) {
+    dstp[j] = reinterpret_cast<uint64_t>(d.dst[j + 1]);
+    in_vgpr(dstp[j]);
+  }
   // (unit indices fit 32 bits: the count array has one entry per unit)
   const uint32_t G32 = uint32_t(G_);
   const uint32_t gi = 16 * uint32_t(uwid) + uint32_t(lane & 15);
