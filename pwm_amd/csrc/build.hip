@@ -3597,10 +3597,13 @@ bool wide_gather_disabled() {
 }
 
 // 128-word steps per wave of the wide stitch (multiple of 4; PWMHIP_WIDE_STEPS).
-uint32_t wide_gather_steps() {
+// dflt: 8 for one-pass builds, 4 for the passes of larger alphabets (shorter
+// runs: more waves in flight paid off, 2.90 against 2.91 ms on the 20-level
+// Zipf matrix; the 3-level DNA stitch ran 0.128 against 0.137 ms at 8).
+uint32_t wide_gather_steps(uint32_t dflt) {
   const char* e = std::getenv("PWMHIP_WIDE_STEPS");  // read at every call
-  const long x = e ? std::atol(e) : 8;
-  return uint32_t(x >= 4 && x <= 256 ? (x + 3) / 4 * 4 : 8);
+  const long x = e ? std::atol(e) : long(dflt);
+  return uint32_t(x >= 4 && x <= 256 ? (x + 3) / 4 * 4 : dflt);
 }
 
 // PWMHIP_SCATTER_RANKED=1: the symbol reorder ranks 64 symbols per step
@@ -3663,7 +3666,7 @@ void launch_super(pwmhip_ctx* ctx, const K1Params& prm) {
 }  // namespace
 
 uint64_t wide_stitch_first_words(uint64_t max_bits, uint32_t nsub) {
-  const uint32_t nstep = wide_gather_steps();
+  const uint32_t nstep = wide_gather_steps(8);
   const uint64_t wpl = (words_for(max_bits) + 128ull * nstep - 1) / (128ull * nstep);
   return std::max<uint64_t>(1, wpl * nsub);
 }
@@ -3685,7 +3688,7 @@ void launch_wide_stitch(pwmhip_ctx* ctx, const WideStitch& w) {
   }
   wp.units = w.units;
   wp.n = maxb;
-  wp.nstep = wide_gather_steps();
+  wp.nstep = wide_gather_steps(8);
   wp.wpl = (words_for(maxb) + 128ull * wp.nstep - 1) / (128ull * wp.nstep);
   hipStream_t s = ctx->stream;
   hipLaunchKernelGGL(k_wave_starts, dim3(uint32_t((wp.wpl + 255) / 256), w.nsub), dim3(256), 0, s, wp);
@@ -3941,7 +3944,7 @@ void build_passes_device(pwmhip_ctx* ctx, const void* d_text, uint64_t n, uint32
   const uint64_t units_max = (nchunks + 7) / 8;
   const uint64_t tstride =
       sup ? std::max((nchunks + 15) / 16 * (16 * kChunkSyms / 64), units_max * (8 * kChunkSyms / 64)) + 64 : stride;
-  const uint32_t wide_nstep = wide_gather_steps();
+  const uint32_t wide_nstep = wide_gather_steps(4);
   const uint64_t wide_wpl = (words_for(n) + 128ull * wide_nstep - 1) / (128ull * wide_nstep);
   auto carve = [&](arena& A, std::vector<void*>& v) {
     v.clear();
@@ -4226,7 +4229,7 @@ void build_plain_device(pwmhip_ctx* ctx, const void* d_text, uint64_t n, uint32_
   const uint64_t tstride = sup ? units * (unit_syms / 64) + 64 : stride;
   // wide stitch: 128-word steps, wide_nstep per wave, one grid for all sub-levels
   const bool wide = sup && !wide_gather_disabled();
-  const uint32_t wide_nstep = wide_gather_steps();
+  const uint32_t wide_nstep = wide_gather_steps(8);
   const uint64_t wide_wpl = (words_for(n) + 128ull * wide_nstep - 1) / (128ull * wide_nstep);
   // 2 or 3 byte levels: bucket-0 runs written in place by the local build
   // (16-chunk units, offsets in one fused launch, the wide stitch)
