@@ -1281,7 +1281,8 @@ __global__ __launch_bounds__(64 * WAVES) void k_local_super(K1Params p) {
 #pragma unroll
     for (int q = 0; q < NQ; ++q) {
       const uint32_t v = lane < WAVES ? wt[lane][q] : 0u;
-      const uint32_t inc = wave_incl_scan(v);
+      static_assert(WAVES <= 16, "per-wave totals in one row of lanes");
+      const uint32_t inc = row_sum16(v);  // (values in lanes < WAVES: row 0's prefix)
       stot[q] = __builtin_amdgcn_readlane(inc, WAVES - 1);
       spre[q] = __builtin_amdgcn_readlane(inc - v, uwid);
     }
@@ -1604,7 +1605,8 @@ __global__ __launch_bounds__(64 * kSuper) void k_local_direct(K1Params p, Direct
 #pragma unroll
     for (int q = 0; q < NQ; ++q) {
       const uint32_t v = lane < WAVES ? wt[lane][q] : 0u;
-      const uint32_t i2 = wave_incl_scan(v);
+      static_assert(WAVES <= 16, "per-wave totals in one row of lanes");
+      const uint32_t i2 = row_sum16(v);  // (values in lanes < WAVES: row 0's prefix)
       stot[q] = __builtin_amdgcn_readlane(i2, WAVES - 1);
       spre[q] = __builtin_amdgcn_readlane(i2 - v, uwid);
     }
@@ -1886,7 +1888,8 @@ __global__ __launch_bounds__(64 * WAVES) void k_local_super_batch(K1Params p, co
 #pragma unroll
     for (int q = 0; q < NF; ++q) {
       const uint32_t v = lane < WAVES ? wt[lane][q] : 0u;
-      const uint32_t inc = wave_incl_scan(v);
+      static_assert(WAVES <= 16, "per-wave totals in one row of lanes");
+      const uint32_t inc = row_sum16(v);  // (values in lanes < WAVES: row 0's prefix)
       stot[q] = __builtin_amdgcn_readlane(inc, WAVES - 1);
       if (q < NQ) spre[q] = __builtin_amdgcn_readlane(inc - v, uwid);
     }
@@ -2085,7 +2088,8 @@ __global__ __launch_bounds__(64 * kSuper) void k_huff_super(HuffSuperParams hp, 
 #pragma unroll
     for (int q = 0; q < NQ; ++q) {
       const uint32_t v = lane < WAVES ? wt[lane][q] : 0u;
-      const uint32_t inc = wave_incl_scan(v);
+      static_assert(WAVES <= 16, "per-wave totals in one row of lanes");
+      const uint32_t inc = row_sum16(v);  // (values in lanes < WAVES: row 0's prefix)
       stot[q] = __builtin_amdgcn_readlane(inc, WAVES - 1);
       spre[q] = __builtin_amdgcn_readlane(inc - v, uwid);
     }
