@@ -4515,7 +4515,7 @@ struct BatchTune {
 };
 
 const BatchTune kTuneCandidates[] = {{4, 8}, {16, 8}, {64, 8}, {4, 32}, {16, 32}, {64, 32}};
-const BatchTune kTuneCandidatesUnits[] = {{1, 4}, {1, 8}, {1, 16}};
+const BatchTune kTuneCandidatesUnits[] = {{1, 2}, {1, 4}, {1, 8}, {1, 16}};
 
 }  // namespace
 
