@@ -1370,11 +1370,13 @@ struct DirectParams {
                               // bits of dword j - 1, bit 31 of both dwords = published
   unsigned long long* fault;  // nonzero: a wait for a unit's counts gave up
   uint64_t* dst[kMaxGroup];   // final sub-level j at dst[j]
+  uint32_t spin_limit;        // reloads of one entry before its wait gives up
+  uint32_t publish;           // 0 (a test of the give-up path): no unit publishes
 };
 
 constexpr unsigned long long kAggFlag = 0x8000000080000000ull;  // in both dwords: a dword read alone is whole
 constexpr uint32_t kGapEntries = 256;      // gap entries loaded a step ahead (grid <= 257)
-constexpr uint32_t kSpinLimit = 1u << 16;  // reloads of one entry before its wait gives up
+constexpr uint32_t kSpinLimit = 1u << 16;  // reloads of one entry before its wait gives up (default)
 
 __device__ __forceinline__ unsigned long long agg_load(const unsigned long long* p) {
   return __hip_atomic_load(p, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
@@ -1487,7 +1489,7 @@ __global__ __launch_bounds__(64 * kSuper) void k_local_direct(K1Params p, Direct
       uint32_t spins = 0;
 #pragma clang loop unroll(disable)
       while (!(x & 0x80000000u)) {
-        if (gave_up || ++spins > kSpinLimit) {
+        if (gave_up || ++spins > d.spin_limit) {
           gave_up = true;
           atomicOr(d.fault, 1ull);
           break;
@@ -1622,7 +1624,7 @@ __global__ __launch_bounds__(64 * kSuper) void k_local_direct(K1Params p, Direct
       unsigned long long a = 0;
 #pragma unroll
       for (int j = 0; j < NSUB; ++j) a |= (unsigned long long)c0[j] << (32 * j);
-      __hip_atomic_store(d.agg + sc, a | kAggFlag, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+      if (d.publish) __hip_atomic_store(d.agg + sc, a | kAggFlag, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
     }
     // the run starts of the unit built two steps ago
     uint64_t D[NSUB];
@@ -4356,6 +4358,13 @@ void build_plain_device(pwmhip_ctx* ctx, const void* d_text, uint64_t n, uint32_
       dp.agg = agg;
       dp.fault = reinterpret_cast<unsigned long long*>(zeros_dev + 63);
       for (uint32_t j = 1; j < L; ++j) dp.dst[j] = ws->d_words + ws->level_offset[j];
+      // PWMHIP_TEST_UNPUBLISHED=1 (tests of the give-up path): no unit
+      // publishes its counts and a wait gives up after 16 reloads, so the
+      // build ends with PWMHIP_ERR_TIMEOUT and every kernel exits normally
+      const char* tu = std::getenv("PWMHIP_TEST_UNPUBLISHED");  // read at every call
+      const bool unpublished = tu && tu[0] == '1';
+      dp.publish = unpublished ? 0u : 1u;
+      dp.spin_limit = unpublished ? 16u : kSpinLimit;
       if (L == 2) launch_direct<2>(ctx, k1, dp);
       else launch_direct<3>(ctx, k1, dp);
     } else if (sup) {

@@ -93,3 +93,24 @@ def test_inplace_equals_stitch_all(monkeypatch):
             assert np.array_equal(a.zeros(), b.zeros()), (L, kind)
             a.free()
             b.free()
+
+
+@pytest.mark.gpu
+def test_inplace_wait_gives_up(monkeypatch):
+    """The bounded wait for earlier units' counts: with no unit publishing
+    (PWMHIP_TEST_UNPUBLISHED=1, waits give up after 16 reloads) the build
+    ends with the timeout status instead of hanging, and the next build on
+    the same context is exact again."""
+    t = pwm_amd.generate("dna", 0xD5, 40 * UNIT + 77)
+    ctx = pwm_amd.Context(0)
+    try:
+        monkeypatch.setenv("PWMHIP_TEST_UNPUBLISHED", "1")
+        with pytest.raises(pwm_amd.PwmError) as e:
+            pwm_amd.build(t, 3, "ppc", False, ctx=ctx)
+        assert e.value.status == pwm_amd.ERR_TIMEOUT
+        monkeypatch.delenv("PWMHIP_TEST_UNPUBLISHED")
+        ws = pwm_amd.build(t, 3, "pps", False, ctx=ctx)
+        _check(ws, t, 3, False, "after the timeout")
+        ws.free()
+    finally:
+        ctx.close()
